@@ -1,0 +1,83 @@
+/*
+ * Plain-C / FFI view of the asdFft* entry points, plus this library's extensions.
+ *
+ * The asdFft* functions below are the same unmangled symbols as include/fft_api.h (the reference's
+ * include/fft_api.h:48-96); where the C++ header takes a reference (asdFftHandle &, size_t &,
+ * asdFftType &) this view takes a pointer, which is the same machine-level argument. Include one
+ * of the two headers in a translation unit, not both: in C++ the extension block below is also
+ * available after including fft_api.h (it is guarded for that case).
+ *
+ * Enum values (include/fft_api.h:26-44 in the reference):
+ *   type      C2C = 0x10, C2R = 0x11, R2C = 0x12, STFT_* = 0x20-0x22, C2C_SEP = 0x30
+ *   direction FORWARD = 0x10, INVERSE = 0x11
+ *   dimType   HORIZONTAL = 0x10, VERTICAL = 0x11
+ * Status codes: 0 success, 100000 invalid parameter, 200001 not supported, 500000 internal error.
+ */
+#ifndef SIP_FFT_H
+#define SIP_FFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+#ifdef __cplusplus
+#include "fft_api.h"
+extern "C" {
+#else
+typedef struct aclTensor aclTensor;
+typedef void *asdFftHandle;
+typedef int32_t AspbStatus;
+
+enum { ASCEND_FFT_C2C = 0x10, ASCEND_FFT_C2R = 0x11, ASCEND_FFT_R2C = 0x12, ASCEND_STFT_C2C = 0x20,
+       ASCEND_STFT_C2R = 0x21, ASCEND_STFT_R2C = 0x22, ASCEND_FFT_C2C_SEP = 0x30 };
+enum { ASCEND_FFT_FORWARD = 0x10, ASCEND_FFT_INVERSE = 0x11 };
+enum { ASCEND_FFT_HORIZONTAL = 0x10, ASCEND_FFT_VERTICAL = 0x11 };
+
+AspbStatus asdFftCreate(asdFftHandle *handle);
+AspbStatus asdFftSetStream(asdFftHandle handle, void *stream);
+AspbStatus asdFftMakePlan1D(asdFftHandle handle, int64_t fftSize, int32_t fftType, int32_t direction,
+                            int64_t batchSize, int32_t dimType);
+AspbStatus asdFftMakePlan2D(asdFftHandle handle, int64_t fftSizeX, int64_t fftSizeY, int32_t fftType,
+                            int32_t direction, int32_t batchSize);
+AspbStatus asdFftMakePlan3D(asdFftHandle handle, int64_t fftSizeX, int64_t fftSizeY, int64_t fftSizeZ,
+                            int32_t fftType, int32_t direction, int32_t batchSize);
+AspbStatus asdFftGetWorkspaceSize(asdFftHandle handle, size_t *workspaceSize);
+AspbStatus asdFftSetWorkspace(asdFftHandle handle, void *workspace);
+AspbStatus asdFftExecC2C(asdFftHandle handle, const aclTensor *input, const aclTensor *output);
+AspbStatus asdFftExecC2R(asdFftHandle handle, const aclTensor *input, const aclTensor *output);
+AspbStatus asdFftExecR2C(asdFftHandle handle, const aclTensor *input, const aclTensor *output);
+AspbStatus asdFftExecC2CSeparated(asdFftHandle handle, const aclTensor *inputReal, const aclTensor *inputImag,
+                                  const aclTensor *outputReal, const aclTensor *outputImag);
+AspbStatus asdFftIstftMakePlan(asdFftHandle handle, const aclTensor *input, int64_t nFft, int64_t hopLengthOpt,
+                               int64_t winLengthOpt, bool center, bool normalized, bool onesidedOpt,
+                               int64_t lengthOpt, bool returnComplex);
+AspbStatus asdFftExecIstft(asdFftHandle handle, const aclTensor *input, const aclTensor *windowOpt,
+                           const aclTensor *output);
+AspbStatus asdFftSynchronize(asdFftHandle handle);
+AspbStatus asdFftGetType(asdFftHandle handle, int32_t *fftType);
+AspbStatus asdFftDestroy(asdFftHandle handle);
+#endif
+
+/* ------------------------------------------------------------------ extensions (no counterpart
+ * in the reference; used by the tests, the benchmark and the batching / serving layers) */
+
+/* Writes a JSON description of a handle's plan (stages, kernels, radices, workspace, and the core
+ * the reference would have selected) into buf. Returns 0, 100000 on a bad handle, or 100000 if
+ * cap is too small (then *needed holds the size including the terminating NUL). */
+int32_t sipFftPlanDescribe(void *handle, char *buf, size_t cap, size_t *needed);
+
+/* Same description for a plan that is only laid out on the host (no device memory, no GPU
+ * needed): ndim 1..3, dims[ndim], type/direction/dimType as for asdFftMakePlan*. */
+int32_t sipFftPlanDescribeDry(int32_t ndim, const int64_t *dims, int32_t fftType, int32_t direction,
+                              int64_t batchSize, int32_t dimType, char *buf, size_t cap, size_t *needed);
+
+/* Library version string. */
+const char *sipFftVersion(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

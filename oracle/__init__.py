@@ -1,0 +1,217 @@
+"""FFT oracle (test infrastructure only).
+
+The product library never imports this package. Only ``tests/``, ``__graft_entry__.smoke()`` and
+the CPU-baseline leg of ``bench.py`` use it, as the checker. It wraps ``oracle/fft_oracle.c``
+(built into ``oracle/liboracle.so``): an FP64 Stockham FFT over the prime factorisation of N, an
+O(N^2) long-double DFT, PCG32 seeded input generators, the rel-L2 / max-abs error report, and an
+FP32 CPU FFT of the same staged algorithm (the CPU baseline).
+
+Transform semantics restated from the reference (see the header of fft_oracle.c for file:line):
+forward uses e^{-2 pi i kn/N}, inverse e^{+2 pi i kn/N} with no 1/N scaling; R2C returns bins
+0..N/2; C2R transforms the Hermitian extension of N/2+1 bins and keeps the real part; 2D/3D
+transforms are separable, R2C doing the real axis (last) first and C2R last.
+
+Every function here is pinned by tests/test_oracle.py against closed forms (impulse, constant,
+pure tones, the ramp the reference's examples feed in), numpy.fft, and the brute-force DFT.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+FORWARD = -1
+INVERSE = +1
+
+
+def build() -> str:
+    """Compile liboracle.so from fft_oracle.c (gcc + OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        i64, dp, fp = ctypes.c_int64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)
+        L.oracle_fill_uniform.argtypes = [fp, i64, ctypes.c_uint64, ctypes.c_uint64]
+        L.oracle_fill_gaussian.argtypes = [fp, i64, ctypes.c_uint64, ctypes.c_uint64]
+        L.oracle_c2c_f64.argtypes = [dp, i64, i64, ctypes.c_int]
+        L.oracle_c2c_axis_f64.argtypes = [dp, i64, i64, i64, ctypes.c_int]
+        L.oracle_r2c_f64.argtypes = [dp, dp, i64, i64, ctypes.c_int]
+        L.oracle_c2r_f64.argtypes = [dp, dp, i64, i64, ctypes.c_int]
+        L.oracle_dft_ld.argtypes = [dp, dp, i64, ctypes.c_int]
+        L.oracle_error_f32.argtypes = [fp, dp, i64, dp, dp]
+        L.oracle_c2c_f32.argtypes = [fp, fp, i64, i64, ctypes.c_int, ctypes.c_int]
+        L.oracle_max_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"oracle {what} failed with code {rc}")
+
+
+# ------------------------------------------------------------------ seeded inputs
+
+def uniform(count: int, seed: int, stream: int = 0) -> np.ndarray:
+    """`count` float32 values uniform in [-1, 1) from PCG32(seed, stream)."""
+    out = np.empty(int(count), dtype=np.float32)
+    lib().oracle_fill_uniform(_fp(out), out.size, seed, stream)
+    return out
+
+
+def gaussian(count: int, seed: int, stream: int = 0) -> np.ndarray:
+    """`count` float32 N(0,1) values (Box-Muller in FP64) from PCG32(seed, stream)."""
+    out = np.empty(int(count), dtype=np.float32)
+    lib().oracle_fill_gaussian(_fp(out), out.size, seed, stream)
+    return out
+
+
+def uniform_complex(shape, seed: int, stream: int = 0) -> np.ndarray:
+    """complex64 array whose interleaved re/im floats are i.i.d. uniform[-1,1)."""
+    shape = tuple(int(s) for s in np.atleast_1d(shape))
+    n = int(np.prod(shape))
+    return uniform(2 * n, seed, stream).view(np.complex64).reshape(shape)
+
+
+def tones_complex(shape, seed: int, ntones: int = 3, amplitude: float = 1e3) -> np.ndarray:
+    """Gaussian N(0,1) complex64 plus `ntones` pure tones of the given amplitude at seeded random
+    bins of every transform (the precision-form input distribution). The tones are added over the
+    full multi-dimensional index (one random frequency vector per tone, shared by the batch)."""
+    shape = tuple(int(s) for s in shape)
+    x = gaussian(2 * int(np.prod(shape)), seed).view(np.complex64).reshape(shape).astype(np.complex128)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dims = shape[1:]
+    grids = np.meshgrid(*[np.arange(d) for d in dims], indexing="ij")
+    for _ in range(ntones):
+        k = [int(rng.integers(0, d)) for d in dims]
+        phase = sum(2.0 * np.pi * ki * g / d for ki, g, d in zip(k, grids, dims))
+        x += amplitude * np.exp(1j * phase)[None, ...]
+    return x.astype(np.complex64)
+
+
+# ------------------------------------------------------------------ FP64 transforms
+
+def c2c(x: np.ndarray, sign: int = FORWARD) -> np.ndarray:
+    """FP64 C2C along the last axis (batch = all leading axes)."""
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    n = a.shape[-1]
+    _check(lib().oracle_c2c_f64(_dp(a), n, a.size // n if n else 0, sign), "c2c")
+    return a
+
+
+def c2c_axis(x: np.ndarray, axis: int, sign: int = FORWARD) -> np.ndarray:
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    axis = axis % a.ndim
+    outer = int(np.prod(a.shape[:axis]))
+    inner = int(np.prod(a.shape[axis + 1:]))
+    _check(lib().oracle_c2c_axis_f64(_dp(a), outer, a.shape[axis], inner, sign), "c2c_axis")
+    return a
+
+
+def c2c_nd(x: np.ndarray, ndim: int, sign: int = FORWARD) -> np.ndarray:
+    """C2C over the last `ndim` axes."""
+    a = np.asarray(x, dtype=np.complex128)
+    for ax in range(a.ndim - ndim, a.ndim):
+        a = c2c_axis(a, ax, sign)
+    return a
+
+
+def r2c(x: np.ndarray, sign: int = FORWARD) -> np.ndarray:
+    a = np.ascontiguousarray(x, dtype=np.float64)
+    n = a.shape[-1]
+    batch = a.size // n
+    out = np.empty(a.shape[:-1] + (n // 2 + 1,), dtype=np.complex128)
+    _check(lib().oracle_r2c_f64(_dp(a), _dp(out), n, batch, sign), "r2c")
+    return out
+
+
+def c2r(x: np.ndarray, n: int, sign: int = INVERSE) -> np.ndarray:
+    a = np.ascontiguousarray(x, dtype=np.complex128)
+    assert a.shape[-1] == n // 2 + 1
+    batch = a.size // a.shape[-1]
+    out = np.empty(a.shape[:-1] + (n,), dtype=np.float64)
+    _check(lib().oracle_c2r_f64(_dp(a), _dp(out), n, batch, sign), "c2r")
+    return out
+
+
+def r2c_nd(x: np.ndarray, ndim: int, sign: int = FORWARD) -> np.ndarray:
+    """R2C over the last `ndim` axes: real transform on the last axis, then C2C on the others."""
+    a = r2c(x, sign)
+    for ax in range(a.ndim - ndim, a.ndim - 1):
+        a = c2c_axis(a, ax, sign)
+    return a
+
+
+def c2r_nd(x: np.ndarray, n_last: int, ndim: int, sign: int = INVERSE) -> np.ndarray:
+    """C2R over the last `ndim` axes: C2C on the leading axes, then C2R on the last."""
+    a = np.asarray(x, dtype=np.complex128)
+    for ax in range(a.ndim - ndim, a.ndim - 1):
+        a = c2c_axis(a, ax, sign)
+    return c2r(a, n_last, sign)
+
+
+def dft_ld(x: np.ndarray, sign: int = FORWARD) -> np.ndarray:
+    """O(N^2) long-double DFT of a single 1D sequence (rounded to FP64 on return)."""
+    a = np.ascontiguousarray(x, dtype=np.complex128).ravel()
+    out = np.empty_like(a)
+    _check(lib().oracle_dft_ld(_dp(a), _dp(out), a.size, sign), "dft_ld")
+    return out
+
+
+# ------------------------------------------------------------------ error report
+
+def error(got: np.ndarray, ref: np.ndarray):
+    """(rel-L2, max-abs) of an FP32 result against an FP64 reference of the same shape."""
+    g = np.ascontiguousarray(got)
+    r = np.ascontiguousarray(ref)
+    if np.iscomplexobj(g):
+        g = g.astype(np.complex64, copy=False).view(np.float32)
+    else:
+        g = g.astype(np.float32, copy=False)
+    if np.iscomplexobj(r):
+        r = r.astype(np.complex128, copy=False).view(np.float64)
+    else:
+        r = r.astype(np.float64, copy=False)
+    g = g.ravel()
+    r = r.ravel()
+    assert g.size == r.size, (g.size, r.size)
+    rel = ctypes.c_double()
+    mx = ctypes.c_double()
+    lib().oracle_error_f32(_fp(g), _dp(r), g.size, ctypes.byref(rel), ctypes.byref(mx))
+    return rel.value, mx.value
+
+
+# ------------------------------------------------------------------ FP32 CPU baseline
+
+def c2c_f32(x: np.ndarray, sign: int = FORWARD, threads: int = 0) -> np.ndarray:
+    """FP32 C2C along the last axis with the same staged algorithm (CPU baseline)."""
+    a = np.ascontiguousarray(x, dtype=np.complex64)
+    n = a.shape[-1]
+    out = np.empty_like(a)
+    _check(lib().oracle_c2c_f32(_fp(a.view(np.float32)), _fp(out.view(np.float32)), n, a.size // n, sign, threads),
+           "c2c_f32")
+    return out
+
+
+def max_threads() -> int:
+    return int(lib().oracle_max_threads())

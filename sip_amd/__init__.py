@@ -1,0 +1,267 @@
+"""sip_amd: MI355X-native FFT library with the reference's asdFft* API.
+
+The product is the C-ABI shared library ``sip_amd/libsip_fft.so`` (hand-written gfx950 HIP
+kernels behind the asdFft* entry points of the reference's include/fft_api.h). This module is a
+thin ctypes binding that mirrors that interface for Python callers and tests: the same entry
+point names, argument meaning and status codes (see ``include/sip_fft.h``). Device memory and
+streams come from PyTorch (plumbing only); every transform runs in the HIP library, and loading
+fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsip_fft.so")
+
+# enums (reference include/fft_api.h:26-44)
+ASCEND_FFT_C2C = 0x10
+ASCEND_FFT_C2R = 0x11
+ASCEND_FFT_R2C = 0x12
+ASCEND_STFT_C2C = 0x20
+ASCEND_STFT_C2R = 0x21
+ASCEND_STFT_R2C = 0x22
+ASCEND_FFT_C2C_SEP = 0x30
+ASCEND_FFT_FORWARD = 0x10
+ASCEND_FFT_INVERSE = 0x11
+ASCEND_FFT_HORIZONTAL = 0x10
+ASCEND_FFT_VERTICAL = 0x11
+
+# status codes (reference core/utils/include/utils/aspb_status.h:16-27)
+ACL_SUCCESS = 0
+ACL_ERROR_INVALID_PARAM = 100000
+ACL_ERROR_OP_INPUT_NOT_MATCH = 100021
+ACL_ERROR_OP_OUTPUT_NOT_MATCH = 100022
+ACL_ERROR_UNSUPPORTED_DATA_TYPE = 100026
+ACL_ERROR_FORMAT_NOT_MATCH = 100027
+ACL_ERROR_API_NOT_SUPPORT = 200001
+ACL_ERROR_INTERNAL_ERROR = 500000
+
+# aclDataType values used on the path
+ACL_FLOAT = 0
+ACL_COMPLEX64 = 16
+ACL_FORMAT_ND = 2
+
+EXPORTED_SYMBOLS = [
+    "asdFftCreate", "asdFftSetStream", "asdFftMakePlan1D", "asdFftMakePlan2D", "asdFftMakePlan3D",
+    "asdFftGetWorkspaceSize", "asdFftSetWorkspace", "asdFftExecC2C", "asdFftExecC2R", "asdFftExecR2C",
+    "asdFftExecC2CSeparated", "asdFftIstftMakePlan", "asdFftExecIstft", "asdFftSynchronize", "asdFftGetType",
+    "asdFftDestroy", "aclCreateTensor", "aclDestroyTensor", "aclGetDataType", "aclGetViewShape",
+    "aclFreeViewShape", "aclGetTensorData", "sipFftPlanDescribe", "sipFftPlanDescribeDry", "sipFftVersion",
+]
+
+_lib = None
+
+
+class SipFftError(RuntimeError):
+    def __init__(self, what, status):
+        super().__init__(f"{what} failed with status {status}")
+        self.status = status
+
+
+def lib():
+    """Loads libsip_fft.so; raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m sip_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64, sz = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_size_t
+    P = ctypes.POINTER
+    sig = {
+        "asdFftCreate": [P(vp)],
+        "asdFftSetStream": [vp, vp],
+        "asdFftMakePlan1D": [vp, i64, i32, i32, i64, i32],
+        "asdFftMakePlan2D": [vp, i64, i64, i32, i32, i32],
+        "asdFftMakePlan3D": [vp, i64, i64, i64, i32, i32, i32],
+        "asdFftGetWorkspaceSize": [vp, P(sz)],
+        "asdFftSetWorkspace": [vp, vp],
+        "asdFftExecC2C": [vp, vp, vp],
+        "asdFftExecC2R": [vp, vp, vp],
+        "asdFftExecR2C": [vp, vp, vp],
+        "asdFftExecC2CSeparated": [vp, vp, vp, vp, vp],
+        "asdFftIstftMakePlan": [vp, vp, i64, i64, i64, ctypes.c_bool, ctypes.c_bool, ctypes.c_bool, i64,
+                                ctypes.c_bool],
+        "asdFftExecIstft": [vp, vp, vp, vp],
+        "asdFftSynchronize": [vp],
+        "asdFftGetType": [vp, P(i32)],
+        "asdFftDestroy": [vp],
+        "sipFftPlanDescribe": [vp, ctypes.c_char_p, sz, P(sz)],
+        "sipFftPlanDescribeDry": [i32, P(i64), i32, i32, i64, i32, ctypes.c_char_p, sz, P(sz)],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = i32
+    L.aclCreateTensor.argtypes = [P(i64), ctypes.c_uint64, ctypes.c_int, P(i64), i64, ctypes.c_int, P(i64),
+                                  ctypes.c_uint64, vp]
+    L.aclCreateTensor.restype = vp
+    L.aclDestroyTensor.argtypes = [vp]
+    L.aclDestroyTensor.restype = ctypes.c_int
+    L.aclGetDataType.argtypes = [vp, P(ctypes.c_int)]
+    L.aclGetDataType.restype = ctypes.c_int
+    L.aclGetViewShape.argtypes = [vp, P(P(i64)), P(ctypes.c_uint64)]
+    L.aclGetViewShape.restype = ctypes.c_int
+    L.aclFreeViewShape.argtypes = [P(i64)]
+    L.aclFreeViewShape.restype = None
+    L.aclGetTensorData.argtypes = [vp]
+    L.aclGetTensorData.restype = vp
+    L.sipFftVersion.argtypes = []
+    L.sipFftVersion.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _check(status, what):
+    if status != ACL_SUCCESS:
+        raise SipFftError(what, status)
+
+
+class AclTensor:
+    """aclTensor descriptor of a device buffer (address, shape, dtype); owns only the descriptor."""
+
+    def __init__(self, data_ptr: int, shape, dtype: int):
+        shape = [int(s) for s in shape]
+        arr = (ctypes.c_int64 * max(1, len(shape)))(*shape)
+        strides = []
+        acc = 1
+        for s in reversed(shape):
+            strides.append(acc)
+            acc *= s
+        strides = list(reversed(strides))
+        st = (ctypes.c_int64 * max(1, len(shape)))(*strides)
+        self.ptr = lib().aclCreateTensor(arr, len(shape), dtype, st, 0, ACL_FORMAT_ND, arr, len(shape),
+                                         ctypes.c_void_p(int(data_ptr)))
+        if not self.ptr:
+            raise RuntimeError("aclCreateTensor failed")
+
+    @classmethod
+    def from_torch(cls, t):
+        import torch
+        if not t.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        if t.dtype == torch.complex64:
+            dt = ACL_COMPLEX64
+        elif t.dtype == torch.float32:
+            dt = ACL_FLOAT
+        else:
+            raise ValueError(f"unsupported dtype {t.dtype}")
+        return cls(t.data_ptr(), t.shape, dt)
+
+    def close(self):
+        if self.ptr:
+            lib().aclDestroyTensor(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FftHandle:
+    """One asdFftHandle: Create -> MakePlan -> workspace -> SetStream -> Exec -> Synchronize -> Destroy,
+    the call order of the reference's examples (example/A2/FFT/1d/example_acl_fft_c2c_1d.cpp:121-145)."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(lib().asdFftCreate(ctypes.byref(h)), "asdFftCreate")
+        self.h = h
+        self._ws = None
+
+    # -- planning
+    def make_plan_1d(self, n, fft_type, direction, batch, dim_type=ASCEND_FFT_HORIZONTAL):
+        return lib().asdFftMakePlan1D(self.h, n, fft_type, direction, batch, dim_type)
+
+    def make_plan_2d(self, x, y, fft_type, direction, batch):
+        return lib().asdFftMakePlan2D(self.h, x, y, fft_type, direction, batch)
+
+    def make_plan_3d(self, x, y, z, fft_type, direction, batch):
+        return lib().asdFftMakePlan3D(self.h, x, y, z, fft_type, direction, batch)
+
+    def workspace_size(self) -> int:
+        n = ctypes.c_size_t()
+        _check(lib().asdFftGetWorkspaceSize(self.h, ctypes.byref(n)), "asdFftGetWorkspaceSize")
+        return n.value
+
+    def alloc_workspace(self, device="cuda"):
+        """Allocates the workspace with torch (device memory) and binds it to the handle."""
+        import torch
+        n = self.workspace_size()
+        if n > 0:
+            self._ws = torch.empty(n, dtype=torch.uint8, device=device)
+            _check(lib().asdFftSetWorkspace(self.h, ctypes.c_void_p(self._ws.data_ptr())), "asdFftSetWorkspace")
+        return n
+
+    def set_stream(self, stream_ptr: int):
+        return lib().asdFftSetStream(self.h, ctypes.c_void_p(int(stream_ptr)))
+
+    def use_current_torch_stream(self):
+        import torch
+        s = torch.cuda.current_stream().cuda_stream
+        if s:
+            _check(self.set_stream(s), "asdFftSetStream")
+
+    # -- execution on torch tensors (no copies; descriptors only)
+    def _exec(self, fn, x, y):
+        tx, ty = AclTensor.from_torch(x), AclTensor.from_torch(y)
+        try:
+            return fn(self.h, ctypes.c_void_p(tx.ptr), ctypes.c_void_p(ty.ptr))
+        finally:
+            tx.close()
+            ty.close()
+
+    def exec_c2c(self, x, y):
+        return self._exec(lib().asdFftExecC2C, x, y)
+
+    def exec_r2c(self, x, y):
+        return self._exec(lib().asdFftExecR2C, x, y)
+
+    def exec_c2r(self, x, y):
+        return self._exec(lib().asdFftExecC2R, x, y)
+
+    def synchronize(self):
+        return lib().asdFftSynchronize(self.h)
+
+    def get_type(self):
+        t = ctypes.c_int32()
+        st = lib().asdFftGetType(self.h, ctypes.byref(t))
+        return st, t.value
+
+    def describe(self) -> dict:
+        return json.loads(_describe(lambda buf, cap, need: lib().sipFftPlanDescribe(self.h, buf, cap, need)))
+
+    def destroy(self):
+        if self.h is not None:
+            st = lib().asdFftDestroy(self.h)
+            self.h = None
+            self._ws = None
+            return st
+        return ACL_SUCCESS
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def _describe(call) -> str:
+    need = ctypes.c_size_t()
+    call(None, 0, ctypes.byref(need))
+    buf = ctypes.create_string_buffer(need.value)
+    st = call(buf, need.value, ctypes.byref(need))
+    _check(st, "sipFftPlanDescribe")
+    return buf.value.decode()
+
+
+def describe_dry(dims, fft_type, direction, batch, dim_type=ASCEND_FFT_HORIZONTAL) -> dict:
+    """Host-only plan layout (no GPU): stages, kernels, workspace, and the reference's step list."""
+    dims = [int(d) for d in dims]
+    arr = (ctypes.c_int64 * len(dims))(*dims)
+    return json.loads(_describe(lambda buf, cap, need: lib().sipFftPlanDescribeDry(
+        len(dims), arr, fft_type, direction, batch, dim_type, buf, cap, need)))

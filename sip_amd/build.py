@@ -1,0 +1,107 @@
+"""Builds libsip_fft.so (the C-ABI FFT library) for gfx950 with hipcc, in-tree.
+
+Steps: generate the Stockham leaf-kernel instances (csrc/kernels/gen_instances.py) into
+sip_amd/_gen/, compile every .hip / .cpp to an object in parallel, link sip_amd/libsip_fft.so.
+Run `python -m sip_amd.build` (or call build()). Objects are cached by a hash of their inputs
+unless force=True.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import glob
+import hashlib
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+GEN = os.path.join(HERE, "_gen")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libsip_fft.so")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+CLANGXX = os.path.join(ROCM, "llvm", "bin", "clang++")
+ARCH = "gfx950"
+NPARTS = 16
+
+INCLUDES = ["-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(CSRC, "kernels"), "-I" + GEN,
+            "-I" + os.path.join(CSRC, "host")]
+# Packed FP32 instructions are disabled for all device code.
+DEVICE_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+                "-packed-fp32-ops", "-Wno-unused-command-line-argument"] + INCLUDES
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"),
+              "-Wall", "-Wno-unused-parameter"] + INCLUDES
+
+
+def _headers():
+    hs = []
+    for pat in ("include/**/*.h", "sip_amd/csrc/**/*.h"):
+        hs += glob.glob(os.path.join(ROOT, pat), recursive=True)
+    hs += glob.glob(os.path.join(GEN, "*.inc"))
+    return sorted(hs)
+
+
+def _digest(src, flags, headers):
+    h = hashlib.sha256()
+    for p in [src] + headers:
+        with open(p, "rb") as fh:
+            h.update(p.encode())
+            h.update(fh.read())
+    h.update(" ".join(flags).encode())
+    return h.hexdigest()
+
+
+def _compile(job):
+    src, obj, cmd, dig, force = job
+    stamp = obj + ".sha"
+    if not force and os.path.exists(obj) and os.path.exists(stamp) and open(stamp).read() == dig:
+        return src, None
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        return src, r.stderr[-4000:]
+    with open(stamp, "w") as fh:
+        fh.write(dig)
+    return src, None
+
+
+def build(force: bool = True, jobs: int | None = None, verbose: bool = False) -> str:
+    os.makedirs(GEN, exist_ok=True)
+    os.makedirs(OBJ, exist_ok=True)
+    subprocess.run([sys.executable, os.path.join(CSRC, "kernels", "gen_instances.py"), GEN, str(NPARTS)],
+                   check=True, capture_output=not verbose)
+    headers = _headers()
+    work = []
+    dev_srcs = sorted(glob.glob(os.path.join(CSRC, "kernels", "*.hip"))) + sorted(glob.glob(os.path.join(GEN, "*.hip")))
+    host_srcs = sorted(glob.glob(os.path.join(CSRC, "host", "*.cpp")))
+    for src in dev_srcs:
+        obj = os.path.join(OBJ, os.path.basename(src) + ".o")
+        cmd = [HIPCC] + DEVICE_FLAGS + ["-c", src, "-o", obj]
+        work.append((src, obj, cmd, _digest(src, DEVICE_FLAGS, headers), force))
+    for src in host_srcs:
+        obj = os.path.join(OBJ, os.path.basename(src) + ".o")
+        cmd = [CLANGXX] + HOST_FLAGS + ["-c", src, "-o", obj]
+        work.append((src, obj, cmd, _digest(src, HOST_FLAGS, headers), force))
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    errors = []
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        for src, err in ex.map(_compile, work):
+            if err:
+                errors.append((src, err))
+    if errors:
+        msg = "\n".join(f"--- {s}\n{e}" for s, e in errors)
+        raise RuntimeError("libsip_fft build failed:\n" + msg)
+    objs = [w[1] for w in work]
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n" + r.stderr[-4000:])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    t = build(force="--force" in sys.argv, verbose=True)
+    print(t)

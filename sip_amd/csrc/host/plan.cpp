@@ -1,0 +1,772 @@
+// Planner: lays an asdFft* transform out as Stockham / auxiliary kernel stages.
+//
+// Reference behaviour kept: the set of transforms (1D horizontal / vertical, 2D, 3D; C2C, R2C,
+// C2R), their sign and normalisation, the shapes they accept, and the workspace contract
+// (caller-allocated device memory of GetWorkspaceSize bytes, needed only by multi-stage plans;
+// core/fft/fft_api.cpp:699-733, 989-1045). The stage layout is this library's own:
+//   * an axis whose length is a compiled leaf runs as one strided launch, in place when it is not
+//     the first stage (rows for the last axis; column tiles, walked with the unit-stride axis
+//     fastest, for the others) -- no transposes between axes (the reference inserts AsStrided
+//     transposes, fft_api.cpp:478-483, 527-697);
+//   * a longer axis runs as a four-step pair L = L1 * L2 (column FFTs with the W_L^{l2 k1}
+//     twiddle folded into their store, then row FFTs that store transposed), recursively;
+//   * a length with a prime factor above 13 runs through Bluestein's chirp-z convolution with
+//     power-of-two FFTs (the reference's generic matrix core FFTCoreAny, fft_core_any.cpp:282-589,
+//     plays that role there);
+//   * even-length real transforms run a half-length complex FFT with the split post- / pre-process
+//     fused into the same launch when the half length is a leaf (reference A/B tables,
+//     core/fft/fftcore/fft_core_common_func.cpp:569-673).
+#include "plan.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <unordered_map>
+
+namespace sip {
+
+// ====================================================================== tables (host, FP64 -> FP32)
+
+static float2 root_f(int64_t m, int64_t n, int sign)
+{
+    // e^{sign 2 pi i m / n} in long double, exact at the quarter points, rounded once.
+    m %= n;
+    if (m < 0) m += n;
+    long double c, s;
+    if ((4 * m) % n == 0) {
+        const int64_t q = (4 * m) / n;
+        c = (q == 0) ? 1.0L : (q == 2) ? -1.0L : 0.0L;
+        s = (q == 1) ? 1.0L : (q == 3) ? -1.0L : 0.0L;
+    } else {
+        const long double a = 6.283185307179586476925286766559005768L * (long double)m / (long double)n;
+        c = cosl(a);
+        s = sinl(a);
+    }
+    float2 r;
+    r.x = (float)(double)c;
+    r.y = (float)(double)(sign * s);
+    return r;
+}
+
+static TableRef pass_twiddles(const StockhamInstance *in, int sign)
+{
+    TableRef t;
+    t.key = "tw:" + std::to_string(in->L);
+    for (int p = 0; p < in->npass; p++) t.key += "," + std::to_string(in->radix[p]);
+    t.key += ":" + std::to_string(sign);
+    t.gen = [in, sign]() {
+        std::vector<float2> v;
+        int64_t ns = 1;
+        const int64_t L = in->L;
+        for (int p = 0; p < in->npass; p++) {
+            const int64_t R = in->radix[p];
+            const int64_t step = L / (ns * R);
+            for (int64_t k = 0; k < ns; k++)
+                for (int64_t r = 1; r < R; r++) v.push_back(root_f(k * r * step, L, sign));
+            ns *= R;
+        }
+        if (v.empty()) v.push_back(root_f(0, 1, 1));
+        return v;
+    };
+    return t;
+}
+
+static TableRef real_roots(int64_t L, int sign)
+{
+    // w^k = e^{sign 2 pi i k / (2L)}, k = 0..L
+    TableRef t;
+    t.key = "rtw:" + std::to_string(L) + ":" + std::to_string(sign);
+    t.gen = [L, sign]() {
+        std::vector<float2> v((size_t)L + 1);
+        for (int64_t k = 0; k <= L; k++) v[(size_t)k] = root_f(k, 2 * L, sign);
+        return v;
+    };
+    return t;
+}
+
+static int tw4_shift_for(int64_t ntot)
+{
+    int bits = 0;
+    while ((int64_t(1) << bits) < ntot) bits++;
+    return (bits + 1) / 2;
+}
+
+static void four_step_tables(int64_t ntot, int sign, TableRef &lo, TableRef &hi)
+{
+    const int shift = tw4_shift_for(ntot);
+    lo.key = "tw4lo:" + std::to_string(ntot) + ":" + std::to_string(sign);
+    lo.gen = [ntot, sign, shift]() {
+        std::vector<float2> v((size_t)1 << shift);
+        for (int64_t m = 0; m < (int64_t(1) << shift); m++) v[(size_t)m] = root_f(m, ntot, sign);
+        return v;
+    };
+    hi.key = "tw4hi:" + std::to_string(ntot) + ":" + std::to_string(sign);
+    hi.gen = [ntot, sign, shift]() {
+        const int64_t nh = (ntot + (int64_t(1) << shift) - 1) >> shift;
+        std::vector<float2> v((size_t)nh);
+        for (int64_t m = 0; m < nh; m++) v[(size_t)m] = root_f(m << shift, ntot, sign);
+        return v;
+    };
+}
+
+// In-place FP64 radix-2 FFT (host; plan-time table generation only).
+static void fft_pow2_f64(std::vector<double> &re, std::vector<double> &im, int sign)
+{
+    const size_t n = re.size();
+    for (size_t i = 1, j = 0; i < n; i++) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            std::swap(re[i], re[j]);
+            std::swap(im[i], im[j]);
+        }
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        const long double ang = sign * 6.283185307179586476925286766559005768L / (long double)len;
+        std::vector<double> wr(len / 2), wi(len / 2);
+        for (size_t k = 0; k < len / 2; k++) {
+            wr[k] = (double)cosl(ang * (long double)k);
+            wi[k] = (double)sinl(ang * (long double)k);
+        }
+        for (size_t i = 0; i < n; i += len)
+            for (size_t k = 0; k < len / 2; k++) {
+                const double ur = re[i + k], ui = im[i + k];
+                const double xr = re[i + k + len / 2], xi = im[i + k + len / 2];
+                const double vr = xr * wr[k] - xi * wi[k], vi = xr * wi[k] + xi * wr[k];
+                re[i + k] = ur + vr;
+                im[i + k] = ui + vi;
+                re[i + k + len / 2] = ur - vr;
+                im[i + k + len / 2] = ui - vi;
+            }
+    }
+}
+
+static TableRef bluestein_chirp(int64_t N, int sign)
+{
+    // c[n] = e^{sign i pi n^2 / N}, with n^2 reduced mod 2N exactly
+    TableRef t;
+    t.key = "bchirp:" + std::to_string(N) + ":" + std::to_string(sign);
+    t.gen = [N, sign]() {
+        std::vector<float2> v((size_t)N);
+        for (int64_t n = 0; n < N; n++) {
+            const int64_t m = (int64_t)(((__int128)n * n) % (2 * N));
+            v[(size_t)n] = root_f(m, 2 * N, sign);
+        }
+        return v;
+    };
+    return t;
+}
+
+static TableRef bluestein_kernel(int64_t N, int64_t M, int sign)
+{
+    // D = FFT_M(d) / M with d[m] = conj c[|m|] for |m| < N (circularly wrapped), in FP64.
+    TableRef t;
+    t.key = "bkern:" + std::to_string(N) + ":" + std::to_string(M) + ":" + std::to_string(sign);
+    t.gen = [N, M, sign]() {
+        std::vector<double> re((size_t)M, 0.0), im((size_t)M, 0.0);
+        const long double pi = 3.141592653589793238462643383279502884L;
+        for (int64_t n = 0; n < N; n++) {
+            const int64_t m2 = (int64_t)(((__int128)n * n) % (2 * N));
+            const long double a = pi * (long double)m2 / (long double)N;
+            const double cr = (double)cosl(a), ci = (double)(-sign * sinl(a));
+            re[(size_t)n] = cr;
+            im[(size_t)n] = ci;
+            if (n > 0) {
+                re[(size_t)(M - n)] = cr;
+                im[(size_t)(M - n)] = ci;
+            }
+        }
+        fft_pow2_f64(re, im, -1);
+        std::vector<float2> v((size_t)M);
+        for (int64_t k = 0; k < M; k++) {
+            v[(size_t)k].x = (float)(re[(size_t)k] / (double)M);
+            v[(size_t)k].y = (float)(im[(size_t)k] / (double)M);
+        }
+        return v;
+    };
+    return t;
+}
+
+// ====================================================================== layouts
+
+namespace {
+
+struct Dim {
+    int64_t n, s;
+};
+
+// A length-L axis of an array: element j of transform f sits at base(f) + j * es, with the
+// transforms enumerated by `dims` (fastest first).
+struct Lay {
+    BufRef buf;
+    int64_t es = 1;
+    std::vector<Dim> dims;
+};
+
+std::vector<Dim> normalize(const std::vector<Dim> &in)
+{
+    std::vector<Dim> d;
+    for (const Dim &x : in)
+        if (x.n != 1) d.push_back(x);
+    std::vector<Dim> m;
+    for (const Dim &x : d) {
+        if (!m.empty() && x.s == m.back().n * m.back().s) {
+            m.back().n *= x.n;
+        } else {
+            m.push_back(x);
+        }
+    }
+    if (m.empty()) m.push_back({1, 0});
+    return m;
+}
+
+int64_t count(const std::vector<Dim> &d)
+{
+    int64_t c = 1;
+    for (const Dim &x : d) c *= x.n;
+    return c;
+}
+
+// Insert the dim for a split axis so that dims stay ordered by stride (unit stride first).
+size_t insert_pos(const std::vector<Dim> &dims, int64_t stride)
+{
+    size_t pos = 0;
+    while (pos < dims.size() && dims[pos].s < stride) pos++;
+    return pos;
+}
+
+std::vector<Dim> with_dim(std::vector<Dim> dims, size_t pos, Dim d)
+{
+    dims.insert(dims.begin() + (std::ptrdiff_t)pos, d);
+    return dims;
+}
+
+TileDesc to_tile(const Lay &l, bool *ok)
+{
+    std::vector<Dim> d = normalize(l.dims);
+    TileDesc t{};
+    if (d.size() > 3) {
+        *ok = false;
+        return t;
+    }
+    while (d.size() < 3) d.push_back({1, 0});
+    t.n0 = d[0].n;
+    t.n1 = d[1].n;
+    t.s0 = d[0].s;
+    t.s1 = d[1].s;
+    t.s2 = d[2].s;
+    t.es = l.es;
+    t.unit_f = (d[0].s == 1 && d[0].n > 1 && l.es != 1) ? 1 : 0;
+    t.vec2 = (l.es == 1 && (d[0].s % 2 == 0) && (d[1].s % 2 == 0) && (d[2].s % 2 == 0)) ? 1 : 0;
+    t.div_n0 = make_fastdiv((uint32_t)t.n0);
+    t.div_n1 = make_fastdiv((uint32_t)t.n1);
+    return t;
+}
+
+AuxRows to_rows(const Lay &l, bool *ok)
+{
+    std::vector<Dim> d = normalize(l.dims);
+    AuxRows r{};
+    if (d.size() > 2) {
+        *ok = false;
+        return r;
+    }
+    while (d.size() < 2) d.push_back({1, 0});
+    r.n0 = d[0].n;
+    r.s0 = d[0].s;
+    r.s1 = d[1].s;
+    r.es = l.es;
+    r.div_n0 = make_fastdiv((uint32_t)r.n0);
+    return r;
+}
+
+constexpr int64_t kColumnLeafMax = 1024;
+constexpr int64_t kRowLeafMax = 16384;
+
+struct Builder {
+    Plan &p;
+    int64_t ws_count = 0;
+    explicit Builder(Plan &pl) : p(pl) {}
+
+    BufRef new_ws(size_t bytes)
+    {
+        const size_t align = 256;
+        size_t off = (p.ws_bytes + align - 1) / align * align;
+        p.ws_offsets.push_back(off);
+        p.ws_bytes = off + (bytes + align - 1) / align * align;
+        BufRef b;
+        b.kind = BUF_WS;
+        b.ws_index = (int)p.ws_offsets.size() - 1;
+        return b;
+    }
+
+    static bool is_column(const Lay &l)
+    {
+        std::vector<Dim> d = normalize(l.dims);
+        return l.es != 1 && d[0].s == 1 && d[0].n > 1;
+    }
+
+    bool leaf_ok(int64_t L, const Lay &in, const Lay &out) const
+    {
+        if (!find_instance(L)) return false;
+        if (is_column(in) || is_column(out)) return L <= kColumnLeafMax;
+        return L <= kRowLeafMax;
+    }
+
+    void add_stockham(int64_t L, int sign, const Lay &in, const Lay &out, int io_mode, int64_t in_len,
+                      int64_t out_len, int64_t tw4_ntot, int64_t col_div, const std::string &note)
+    {
+        const StockhamInstance *inst = find_instance(L);
+        if (!inst) throw SipError(AsdSip::ACL_ERROR_API_NOT_SUPPORT, "no kernel for length " + std::to_string(L));
+        bool ok = true;
+        Stage st;
+        st.kind = StageKind::Stockham;
+        st.in = in.buf;
+        st.out = out.buf;
+        st.inst = inst;
+        st.sign = sign;
+        StockhamArgs &a = st.sa;
+        std::memset(&a, 0, sizeof(a));
+        a.tin = to_tile(in, &ok);
+        a.tout = to_tile(out, &ok);
+        if (!ok) throw SipError(AsdSip::ACL_ERROR_API_NOT_SUPPORT, "layout has more than 3 batch dimensions");
+        const int64_t nfft = count(normalize(in.dims));
+        if (nfft != count(normalize(out.dims)))
+            throw SipError(AsdSip::ACL_ERROR_INTERNAL_ERROR, "stage in/out transform counts differ");
+        if (nfft >= (int64_t(1) << 31))
+            throw SipError(AsdSip::ACL_ERROR_API_NOT_SUPPORT, "more than 2^31 transforms in one launch");
+        a.nfft = nfft;
+        a.io_mode = io_mode;
+        a.in_len = (int32_t)in_len;
+        a.out_len = (int32_t)out_len;
+        const int64_t maxidx = (io_mode == IO_C2R_EVEN) ? L : L - 1;
+        int lp = (int)(maxidx + (maxidx >> 4)) + 1;
+        if ((lp & 1) == 0) lp++;
+        a.LP = lp;
+        const int T = inst->T;
+        const bool column = a.tin.unit_f || a.tout.unit_f;
+        const size_t per = (size_t)2 * lp * 4 + 24;
+        int F;
+        if (column) {
+            F = std::max(16, 256 / T);
+            F = std::min(F, 256);
+            while (F > 1 && ((size_t)F * per > 80 * 1024 || F * T > 1024)) F /= 2;
+        } else {
+            F = std::max(1, 256 / T);
+            while (F > 1 && ((size_t)F * per > 48 * 1024 || F * T > 1024)) F--;
+        }
+        if (F > nfft) F = (int)nfft;
+        if (F < 1) F = 1;
+        a.F = F;
+        a.div_in_len = make_fastdiv((uint32_t)in_len);
+        a.div_out_len = make_fastdiv((uint32_t)out_len);
+        a.div_F = make_fastdiv((uint32_t)F);
+        a.div_in_vec = make_fastdiv((uint32_t)std::max<int64_t>(1, in_len / 2));
+        a.div_out_vec = make_fastdiv((uint32_t)std::max<int64_t>(1, out_len / 2));
+        a.div_col = make_fastdiv((uint32_t)std::max<int64_t>(1, col_div));
+        if (in_len % 2) a.tin.vec2 = 0;
+        if (out_len % 2) a.tout.vec2 = 0;
+        if (io_mode == IO_R2C_FULL) a.tin.vec2 = 0;
+        if (io_mode == IO_C2R_FULL) a.tout.vec2 = 0;
+        st.tw = pass_twiddles(inst, sign);
+        if (io_mode == IO_R2C_EVEN || io_mode == IO_C2R_EVEN) st.rtw = real_roots(L, sign);
+        if (tw4_ntot) {
+            four_step_tables(tw4_ntot, sign, st.tw4lo, st.tw4hi);
+            a.tw4_ntot = tw4_ntot;
+            const int shift = tw4_shift_for(tw4_ntot);
+            a.tw4_shift = shift;
+            a.tw4_mask = (int64_t(1) << shift) - 1;
+        }
+        st.grid = dim3((unsigned)((nfft + F - 1) / F));
+        st.block = dim3((unsigned)(F * T));
+        st.lds = (size_t)2 * F * lp * sizeof(float) + (size_t)3 * F * sizeof(int64_t);
+        st.note = note;
+        p.stages.push_back(st);
+    }
+
+    void add_aux(int mode, const Lay &in, const Lay &out, int64_t n, int64_t out_len, TableRef tab, float scale,
+                 const std::string &note)
+    {
+        bool ok = true;
+        Stage st;
+        st.kind = StageKind::Aux;
+        st.in = in.buf;
+        st.out = out.buf;
+        AuxArgs &x = st.xa;
+        std::memset(&x, 0, sizeof(x));
+        x.mode = mode;
+        x.rin = to_rows(in, &ok);
+        x.rout = to_rows(out, &ok);
+        if (!ok) throw SipError(AsdSip::ACL_ERROR_API_NOT_SUPPORT, "aux layout has more than 2 batch dimensions");
+        x.rows = count(normalize(in.dims));
+        x.n = n;
+        x.out_len = out_len;
+        x.scale = scale;
+        st.tab0 = tab;
+        st.note = note;
+        p.stages.push_back(st);
+    }
+
+    // Two-leaf split L = L1 * L2 for a four-step pair. Stage B stores transposed (a column tile),
+    // so both factors must be column leaves; the most balanced pair is taken.
+    static bool leaf_split(int64_t L, int64_t &L1, int64_t &L2)
+    {
+        L1 = L2 = 0;
+        double best = 1e30;
+        for (int64_t a = 2; a <= kColumnLeafMax && a < L; a++) {
+            if (L % a) continue;
+            const int64_t b = L / a;
+            if (b > kColumnLeafMax || !find_instance(a) || !find_instance(b)) continue;
+            const double score = std::fabs(std::log((double)a) - std::log((double)b));
+            if (score < best) {
+                best = score;
+                L1 = a;
+                L2 = b;
+            }
+        }
+        return L1 != 0;
+    }
+
+    // Can the axis be done with leaves (possibly split recursively)?
+    static bool decomposable(int64_t L, int depth = 0)
+    {
+        if (L <= kRowLeafMax && find_instance(L)) return true;
+        if (depth > 3) return false;
+        int64_t a, b;
+        if (leaf_split(L, a, b)) return true;
+        for (int64_t l1 = std::min(L, kColumnLeafMax); l1 >= 2; l1--)
+            if (L % l1 == 0 && find_instance(l1) && decomposable(L / l1, depth + 1)) return true;
+        return false;
+    }
+
+    // Complex transform of length L along the axis described by in/out (same transform order).
+    // `in_scratch`: the input buffer may be overwritten.
+    void axis(int64_t L, int sign, const Lay &in, const Lay &out, bool in_scratch, int depth = 0)
+    {
+        if (L == 1) {
+            if (in.buf.kind != out.buf.kind || in.buf.ws_index != out.buf.ws_index)
+                add_aux(AUX_CPLX_TRUNC, in, out, 1, 1, TableRef(), 1.f, "copy (length 1)");
+            return;
+        }
+        if (leaf_ok(L, in, out)) {
+            add_stockham(L, sign, in, out, IO_C2C, L, L, 0, 1, "leaf " + std::to_string(L));
+            return;
+        }
+        if (!decomposable(L)) {
+            bluestein(L, sign, in, out);
+            return;
+        }
+        // Split L = L1 * L2: L1 column FFTs (+ twiddle), then L2 row FFTs with transposed store.
+        int64_t L1 = 0, L2 = 0;
+        if (!leaf_split(L, L1, L2)) {
+            for (int64_t l1 = std::min(L, kColumnLeafMax); l1 >= 2; l1--)
+                if (L % l1 == 0 && find_instance(l1) && decomposable(L / l1, depth + 1)) {
+                    L1 = l1;
+                    L2 = L / l1;
+                    break;
+                }
+        }
+        if (!L1) throw SipError(AsdSip::ACL_ERROR_API_NOT_SUPPORT, "cannot split length " + std::to_string(L));
+        // mid layout: same structure as the input; in place when the input may be overwritten and
+        // the final store does not go back into that same buffer (stage B is not in-place safe)
+        Lay mid = in;
+        const bool same_buf = in.buf.kind == out.buf.kind && (in.buf.kind != BUF_WS || in.buf.ws_index == out.buf.ws_index);
+        if (!in_scratch || same_buf) {
+            int64_t extent = L * in.es;
+            for (const Dim &d : in.dims) extent = std::max(extent, d.s * d.n);
+            mid.buf = new_ws((size_t)extent * 8);
+        }
+        // stage A: element k1 stride es*L2, transforms (l2, batch) with l2 stride es
+        const size_t posA = insert_pos(in.dims, in.es);
+        Lay ain{in.buf, in.es * L2, with_dim(in.dims, posA, {L2, in.es})};
+        Lay aout{mid.buf, mid.es * L2, with_dim(mid.dims, posA, {L2, mid.es})};
+        add_stockham(L1, sign, ain, aout, IO_C2C, L1, L1, L, in.es,
+                     "four-step A: " + std::to_string(L1) + " columns x twiddle W_" + std::to_string(L));
+        // stage B: rows l2 (stride es) of each k1, stored at k1 + L1*k2
+        const size_t posB = insert_pos(out.dims, out.es);
+        Lay bin{mid.buf, mid.es, with_dim(mid.dims, posB, {L1, mid.es * L2})};
+        Lay bout{out.buf, out.es * L1, with_dim(out.dims, posB, {L1, out.es})};
+        axis(L2, sign, bin, bout, true, depth + 1);
+    }
+
+    void bluestein(int64_t N, int sign, const Lay &in, const Lay &out)
+    {
+        int64_t M = 1;
+        while (M < 2 * N - 1) M <<= 1;
+        if (M > (int64_t(1) << 26))
+            throw SipError(AsdSip::ACL_ERROR_API_NOT_SUPPORT, "Bluestein length too large");
+        const int64_t rows = count(normalize(in.dims));
+        BufRef w = new_ws((size_t)rows * M * 8);
+        Lay wl{w, 1, {{rows, M}}};
+        TableRef chirp = bluestein_chirp(N, sign);
+        add_aux(AUX_BLUE_PRE, Lay{in.buf, in.es, in.dims}, wl, N, M, chirp, 1.f, "bluestein chirp " + std::to_string(N));
+        axis(M, -1, wl, wl, true);
+        add_aux(AUX_BLUE_MUL, wl, wl, M, M, bluestein_kernel(N, M, sign), 1.f, "bluestein kernel");
+        axis(M, 1, wl, wl, true);
+        add_aux(AUX_BLUE_POST, wl, Lay{out.buf, out.es, out.dims}, N, N, chirp, 1.f, "bluestein post");
+    }
+
+    // Real-to-complex transform of the last axis (n reals -> n/2+1 bins per row).
+    void r2c_axis(int64_t n, int sign, int64_t rows, const BufRef &src, const BufRef &dst)
+    {
+        const int64_t h = n / 2 + 1;
+        if (n % 2 == 0 && n >= 2) {
+            const int64_t L = n / 2;
+            Lay in{src, 1, {{rows, L}}};
+            Lay out{dst, 1, {{rows, h}}};
+            if (L == 1 || (find_instance(L) && L <= kRowLeafMax)) {
+                if (L == 1) {
+                    // n = 2: X0 = x0 + x1, X1 = x0 - x1 via the post-process with Z = x0 + i x1
+                    BufRef w = new_ws((size_t)rows * 8);
+                    Lay wl{w, 1, {{rows, 1}}};
+                    add_aux(AUX_CPLX_TRUNC, in, wl, 1, 1, TableRef(), 1.f, "r2c n=2 copy");
+                    add_aux(AUX_R2C_POST, wl, out, 1, 2, real_roots(1, sign), 1.f, "r2c n=2 post");
+                } else {
+                    add_stockham(L, sign, in, out, IO_R2C_EVEN, L, h, 0, 1, "r2c even fused " + std::to_string(n));
+                }
+                return;
+            }
+            BufRef w = new_ws((size_t)rows * L * 8);
+            Lay wl{w, 1, {{rows, L}}};
+            axis(L, sign, in, wl, false);
+            add_aux(AUX_R2C_POST, wl, out, L, h, real_roots(L, sign), 1.f, "r2c post");
+            return;
+        }
+        if (find_instance(n) && n <= kRowLeafMax) {
+            Lay in{src, 1, {{rows, n}}};
+            Lay out{dst, 1, {{rows, h}}};
+            add_stockham(n, sign, in, out, IO_R2C_FULL, n, h, 0, 1, "r2c full " + std::to_string(n));
+            return;
+        }
+        BufRef w = new_ws((size_t)rows * n * 8);
+        Lay rin{src, 1, {{rows, n}}};
+        Lay wl{w, 1, {{rows, n}}};
+        add_aux(AUX_REAL_TO_CPLX, rin, wl, n, n, TableRef(), 1.f, "r2c widen");
+        axis(n, sign, wl, wl, true);
+        add_aux(AUX_CPLX_TRUNC, wl, Lay{dst, 1, {{rows, h}}}, n, h, TableRef(), 1.f, "r2c keep half");
+    }
+
+    // Complex-to-real transform of the last axis (n/2+1 bins -> n reals per row).
+    void c2r_axis(int64_t n, int sign, int64_t rows, const BufRef &src, const BufRef &dst)
+    {
+        const int64_t h = n / 2 + 1;
+        if (n % 2 == 0 && n >= 4) {
+            const int64_t L = n / 2;
+            Lay in{src, 1, {{rows, h}}};
+            Lay out{dst, 1, {{rows, L}}};
+            if (find_instance(L) && L <= kRowLeafMax) {
+                add_stockham(L, sign, in, out, IO_C2R_EVEN, h, L, 0, 1, "c2r even fused " + std::to_string(n));
+                return;
+            }
+            BufRef w = new_ws((size_t)rows * L * 8);
+            Lay wl{w, 1, {{rows, L}}};
+            add_aux(AUX_C2R_PRE, in, wl, L, L, real_roots(L, sign), 1.f, "c2r pre");
+            axis(L, sign, wl, out, true);
+            return;
+        }
+        if (n > 1 && find_instance(n) && n <= kRowLeafMax) {
+            Lay in{src, 1, {{rows, h}}};
+            Lay out{dst, 1, {{rows, n}}};
+            add_stockham(n, sign, in, out, IO_C2R_FULL, h, n, 0, 1, "c2r full " + std::to_string(n));
+            return;
+        }
+        BufRef w = new_ws((size_t)rows * n * 8);
+        Lay wl{w, 1, {{rows, n}}};
+        add_aux(AUX_HERM_EXTEND, Lay{src, 1, {{rows, h}}}, wl, n, n, TableRef(), 1.f, "c2r hermitian extend");
+        axis(n, sign, wl, wl, true);
+        add_aux(AUX_CPLX_REAL, wl, Lay{dst, 1, {{rows, n}}}, n, n, TableRef(), 1.f, "c2r real part");
+    }
+};
+
+}  // namespace
+
+// ====================================================================== plan build
+
+void plan_build(Plan &p)
+{
+    p.stages.clear();
+    p.ws_offsets.clear();
+    p.ws_bytes = 0;
+    Builder b(p);
+    const int sign = p.sign();
+    const int nd = (int)p.sizes.size();
+    BufRef IN, OUT;
+    IN.kind = BUF_IN;
+    OUT.kind = BUF_OUT;
+    const int64_t B = p.batch;
+
+    if (nd == 1) {
+        const int64_t N = p.sizes[0];
+        if (p.fftType == kTypeC2C) {
+            if (p.dimType == kDimVertical) {
+                // (N, batch): axis 0 with element stride batch
+                Lay in{IN, B, {{B, 1}}};
+                Lay out{OUT, B, {{B, 1}}};
+                b.axis(N, sign, in, out, false);
+            } else {
+                Lay in{IN, 1, {{B, N}}};
+                Lay out{OUT, 1, {{B, N}}};
+                b.axis(N, sign, in, out, false);
+            }
+        } else if (p.fftType == kTypeR2C) {
+            b.r2c_axis(N, sign, B, IN, OUT);
+        } else if (p.fftType == kTypeC2R) {
+            b.c2r_axis(N, sign, B, IN, OUT);
+        } else {
+            throw SipError(AsdSip::ACL_ERROR_API_NOT_SUPPORT, "fft type not supported");
+        }
+        return;
+    }
+
+    // 2D / 3D: array [B][d0][d1]([d2]), last axis fastest.
+    std::vector<int64_t> d = p.sizes;
+    if (p.fftType == kTypeC2C) {
+        // last axis: rows in -> out, then the other axes in place on out
+        int64_t rows = B;
+        for (int i = 0; i < nd - 1; i++) rows *= d[i];
+        const int64_t nl = d[nd - 1];
+        b.axis(nl, sign, Lay{IN, 1, {{rows, nl}}}, Lay{OUT, 1, {{rows, nl}}}, false);
+        for (int ax = nd - 2; ax >= 0; ax--) {
+            int64_t inner = 1, outer = B;
+            for (int i = ax + 1; i < nd; i++) inner *= d[i];
+            for (int i = 0; i < ax; i++) outer *= d[i];
+            Lay l{OUT, inner, {{inner, 1}, {outer, d[ax] * inner}}};
+            b.axis(d[ax], sign, l, l, true);
+        }
+        return;
+    }
+    if (p.fftType == kTypeR2C) {
+        const int64_t nl = d[nd - 1], h = nl / 2 + 1;
+        int64_t rows = B;
+        for (int i = 0; i < nd - 1; i++) rows *= d[i];
+        b.r2c_axis(nl, sign, rows, IN, OUT);
+        for (int ax = nd - 2; ax >= 0; ax--) {
+            int64_t inner = h, outer = B;
+            for (int i = ax + 1; i < nd - 1; i++) inner *= d[i];
+            for (int i = 0; i < ax; i++) outer *= d[i];
+            Lay l{OUT, inner, {{inner, 1}, {outer, d[ax] * inner}}};
+            b.axis(d[ax], sign, l, l, true);
+        }
+        return;
+    }
+    if (p.fftType == kTypeC2R) {
+        const int64_t nl = d[nd - 1], h = nl / 2 + 1;
+        int64_t rows = B;
+        for (int i = 0; i < nd - 1; i++) rows *= d[i];
+        // complex axes first, from the input into a workspace copy (the input is not modified)
+        BufRef w = b.new_ws((size_t)rows * h * 8);
+        bool first = true;
+        for (int ax = 0; ax <= nd - 2; ax++) {
+            int64_t inner = h, outer = B;
+            for (int i = ax + 1; i < nd - 1; i++) inner *= d[i];
+            for (int i = 0; i < ax; i++) outer *= d[i];
+            Lay lin{first ? IN : w, inner, {{inner, 1}, {outer, d[ax] * inner}}};
+            Lay lout{w, inner, {{inner, 1}, {outer, d[ax] * inner}}};
+            b.axis(d[ax], sign, lin, lout, !first);
+            first = false;
+        }
+        b.c2r_axis(nl, sign, rows, w, OUT);
+        return;
+    }
+    throw SipError(AsdSip::ACL_ERROR_API_NOT_SUPPORT, "fft type not supported");
+}
+
+// ====================================================================== device side
+
+void plan_materialize(Plan &p)
+{
+    SIP_HIP_CHECK(hipGetDevice(&p.device));
+    for (Stage &s : p.stages) {
+        if (s.kind == StageKind::Stockham) {
+            s.tw.dev = coeff_get(s.tw);
+            if (s.rtw.used()) s.rtw.dev = coeff_get(s.rtw);
+            if (s.tw4lo.used()) {
+                s.tw4lo.dev = coeff_get(s.tw4lo);
+                s.tw4hi.dev = coeff_get(s.tw4hi);
+            }
+            s.sa.tw = static_cast<const float2 *>(s.tw.dev->ptr);
+            s.sa.rtw = s.rtw.dev ? static_cast<const float2 *>(s.rtw.dev->ptr) : nullptr;
+            s.sa.tw4_lo = s.tw4lo.dev ? static_cast<const float2 *>(s.tw4lo.dev->ptr) : nullptr;
+            s.sa.tw4_hi = s.tw4hi.dev ? static_cast<const float2 *>(s.tw4hi.dev->ptr) : nullptr;
+            const void *fn = s.sign < 0 ? s.inst->fwd : s.inst->inv;
+            if (s.lds > 64 * 1024)
+                SIP_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s.lds));
+        } else {
+            if (s.tab0.used()) {
+                s.tab0.dev = coeff_get(s.tab0);
+                s.xa.tab0 = static_cast<const float2 *>(s.tab0.dev->ptr);
+            }
+        }
+    }
+}
+
+static void *resolve(const Plan &p, const BufRef &b, const void *in, void *out)
+{
+    if (b.kind == BUF_IN) return const_cast<void *>(in);
+    if (b.kind == BUF_OUT) return out;
+    return static_cast<char *>(p.workspace) + p.ws_offsets[(size_t)b.ws_index];
+}
+
+void plan_execute(Plan &p, const void *in, void *out)
+{
+    for (Stage &s : p.stages) {
+        void *src = resolve(p, s.in, in, out);
+        void *dst = resolve(p, s.out, in, out);
+        if (s.kind == StageKind::Stockham) {
+            StockhamArgs a = s.sa;
+            a.in = src;
+            a.out = dst;
+            if (a.tin.vec2 && (reinterpret_cast<uintptr_t>(src) & 15)) a.tin.vec2 = 0;
+            if (a.tout.vec2 && (reinterpret_cast<uintptr_t>(dst) & 15)) a.tout.vec2 = 0;
+            void *args[] = {&a};
+            const void *fn = s.sign < 0 ? s.inst->fwd : s.inst->inv;
+            SIP_HIP_CHECK(hipLaunchKernel(fn, s.grid, s.block, args, s.lds, p.stream));
+        } else {
+            AuxArgs a = s.xa;
+            a.in = src;
+            a.out = dst;
+            SIP_HIP_CHECK(aux_launch(a, p.stream));
+        }
+    }
+}
+
+// ====================================================================== description
+
+static const char *buf_name(const BufRef &b)
+{
+    return b.kind == BUF_IN ? "in" : b.kind == BUF_OUT ? "out" : "ws";
+}
+
+std::string plan_describe(const Plan &p)
+{
+    std::ostringstream os;
+    os << "{\"type\":" << p.fftType << ",\"direction\":" << p.direction << ",\"sizes\":[";
+    for (size_t i = 0; i < p.sizes.size(); i++) os << (i ? "," : "") << p.sizes[i];
+    os << "],\"batch\":" << p.batch << ",\"dimType\":" << p.dimType << ",\"workspace\":" << p.ws_bytes;
+    os << ",\"stages\":[";
+    for (size_t i = 0; i < p.stages.size(); i++) {
+        const Stage &s = p.stages[i];
+        if (i) os << ",";
+        os << "{\"kind\":\"" << (s.kind == StageKind::Stockham ? "stockham" : "aux") << "\",\"in\":\""
+           << buf_name(s.in) << "\",\"out\":\"" << buf_name(s.out) << "\",\"note\":\"" << s.note << "\"";
+        if (s.kind == StageKind::Stockham) {
+            os << ",\"L\":" << s.inst->L << ",\"T\":" << s.inst->T << ",\"F\":" << s.sa.F << ",\"radices\":[";
+            for (int r = 0; r < s.inst->npass; r++) os << (r ? "," : "") << s.inst->radix[r];
+            os << "],\"nfft\":" << s.sa.nfft << ",\"io\":" << s.sa.io_mode << ",\"lds\":" << s.lds
+               << ",\"grid\":" << s.grid.x << ",\"block\":" << s.block.x << ",\"unit_f_in\":" << s.sa.tin.unit_f
+               << ",\"unit_f_out\":" << s.sa.tout.unit_f << ",\"tw4\":" << s.sa.tw4_ntot;
+        } else {
+            os << ",\"mode\":" << s.xa.mode << ",\"rows\":" << s.xa.rows << ",\"n\":" << s.xa.n
+               << ",\"out_len\":" << s.xa.out_len;
+        }
+        os << "}";
+    }
+    os << "],\"reference_steps\":[";
+    for (size_t i = 0; i < p.ref_steps.size(); i++) os << (i ? "," : "") << "\"" << p.ref_steps[i] << "\"";
+    os << "]}";
+    return os.str();
+}
+
+}  // namespace sip

@@ -1,0 +1,121 @@
+// FFT plan: the host-side state behind an asdFftHandle, and the planner that lays a transform out
+// as a sequence of kernel stages.
+//
+// Reference counterparts: FFTPlan / PlanStep (core/fft/fftplan/include/fftplan/fft_plan.h:43-70),
+// the plan builders addFFTSteps / init2DSteps / init3DSteps (core/fft/fft_api.cpp:485-697) and the
+// workspace contract (fft_api.cpp:699-733, 989-1045). The decomposition differs on purpose: the
+// reference chains whole-array steps (a DFT-matrix core per axis, with AsStrided transposes in
+// between); here every axis is one strided Stockham launch that reads and writes in place, long
+// axes become four-step launch pairs, and lengths with large prime factors use Bluestein.
+#pragma once
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../kernels/aux_args.h"
+#include "../kernels/stockham_args.h"
+#include "common.h"
+
+namespace sip {
+
+constexpr int kTypeC2C = 0x10, kTypeC2R = 0x11, kTypeR2C = 0x12, kTypeC2CSep = 0x30;
+constexpr int kDirForward = 0x10, kDirInverse = 0x11;
+constexpr int kDimHorizontal = 0x10, kDimVertical = 0x11;
+constexpr int64_t kMaxFftSize = int64_t(1) << 27;
+
+// Device-resident coefficient table (twiddles, roots, chirps), shared between plans.
+struct DeviceTable {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int device = -1;
+    ~DeviceTable();
+};
+using TablePtr = std::shared_ptr<DeviceTable>;
+
+// A table is identified by a key (its content is a pure function of the key) and generated on
+// the host in FP64 / long double, rounded once to FP32.
+struct TableRef {
+    std::string key;
+    std::function<std::vector<float2>()> gen;
+    TablePtr dev;
+    bool used() const { return !key.empty(); }
+};
+
+// Returns the cached device table for ref.key, creating it from ref.gen when absent
+// (counterpart of FFTensorCache::getCoeff, core/fft/utils/fftensor_cache.cpp:80-92).
+TablePtr coeff_get(const TableRef &ref);
+size_t coeff_cache_live_tables();
+
+enum BufKind : int { BUF_IN = 0, BUF_OUT = 1, BUF_WS = 2 };
+
+struct BufRef {
+    int kind = BUF_IN;
+    int ws_index = 0;  // which workspace region when kind == BUF_WS
+};
+
+enum class StageKind { Stockham, Aux };
+
+struct Stage {
+    StageKind kind = StageKind::Stockham;
+    BufRef in, out;
+    // Stockham
+    const StockhamInstance *inst = nullptr;
+    int sign = -1;
+    StockhamArgs sa{};
+    dim3 grid, block;
+    size_t lds = 0;
+    TableRef tw, rtw, tw4lo, tw4hi;
+    // Aux
+    AuxArgs xa{};
+    TableRef tab0;
+    std::string note;
+};
+
+struct Plan {
+    int fftType = kTypeC2C;
+    int direction = kDirForward;
+    int dimType = kDimHorizontal;
+    std::vector<int64_t> sizes;   // FFT sizes (1..3 dims)
+    std::vector<int64_t> strides; // 1 unless vertical
+    int64_t batch = 1;
+    hipStream_t stream = nullptr;
+    void *workspace = nullptr;
+    bool initialized = false;
+    bool failed = false;
+    int device = 0;
+
+    std::vector<Stage> stages;
+    std::vector<size_t> ws_offsets;  // byte offset of each workspace region
+    size_t ws_bytes = 0;
+
+    std::vector<std::string> ref_steps;  // the reference's step list for the same call
+    std::mutex mu;
+
+    int sign() const { return direction == kDirInverse ? 1 : -1; }
+};
+
+// Lays the plan out (stages, workspace) from its type / sizes / batch. Host only; throws SipError.
+void plan_build(Plan &p);
+// Creates the device-side coefficient tables and kernel attributes. Throws SipError.
+void plan_materialize(Plan &p);
+// Enqueues the stages on p.stream. in/out are device addresses of the user tensors.
+void plan_execute(Plan &p, const void *in, void *out);
+// JSON description of the stages.
+std::string plan_describe(const Plan &p);
+
+// Kernel registry (generated instances).
+const StockhamInstance *find_instance(int64_t L);
+int64_t instance_count();
+
+// Launchers implemented in the .hip files.
+hipError_t aux_launch(const AuxArgs &a, hipStream_t stream);
+
+// The reference's core decomposition for the same call (restated rules of
+// core/fft/fft_api.cpp:169-245, 383-697 and core/fft/fftcore/select_core.cpp:155-200).
+std::vector<std::string> reference_steps(int fftType, int direction, const std::vector<int64_t> &sizes,
+                                         int64_t batch, int dimType);
+
+}  // namespace sip

@@ -1,0 +1,37 @@
+// Lookup of the compiled Stockham leaf kernels (generated instance tables).
+#include <map>
+
+#include "plan.h"
+
+namespace sip {
+
+#define SIP_STOCKHAM_PART(k)                                \
+    extern const StockhamInstance g_stockham_part_##k[];    \
+    extern const int g_stockham_part_##k##_count;
+#include "stockham_parts.inc"
+#undef SIP_STOCKHAM_PART
+
+static const std::map<int64_t, const StockhamInstance *> &instances()
+{
+    static const std::map<int64_t, const StockhamInstance *> m = [] {
+        std::map<int64_t, const StockhamInstance *> r;
+#define SIP_STOCKHAM_PART(k)                                        \
+    for (int i = 0; i < g_stockham_part_##k##_count; i++)           \
+        r[g_stockham_part_##k[i].L] = &g_stockham_part_##k[i];
+#include "stockham_parts.inc"
+#undef SIP_STOCKHAM_PART
+        return r;
+    }();
+    return m;
+}
+
+const StockhamInstance *find_instance(int64_t L)
+{
+    const auto &m = instances();
+    auto it = m.find(L);
+    return it == m.end() ? nullptr : it->second;
+}
+
+int64_t instance_count() { return (int64_t)instances().size(); }
+
+}  // namespace sip

@@ -1,0 +1,79 @@
+// Launch descriptors shared by the host planner and the gfx950 Stockham kernels.
+//
+// One launch runs `nfft` independent transforms of length L. Each transform is staged in LDS
+// (separate re / im float planes with one pad word every 16 words), goes through the radix passes
+// of the Stockham autosort recursion (the radix sequence is a compile-time property of the kernel
+// instance), and is written back. Transform f is addressed as f = f0 + n0 * f1 (f0 < n0); element j
+// of it sits at  base + f0*s0 + f1*s1 + j*es  (in complex elements for complex data, in floats for
+// real data). With these strides one kernel covers contiguous batches, the column passes of 2D/3D
+// transforms and the passes of a four-step transform, always with coalesced global accesses: the
+// tile is walked along whichever axis has unit stride.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sip {
+
+// Exact unsigned division of n < 2^31 by a runtime constant d >= 1
+// (round-up multiplier: q = (umulhi(n, m) + n) >> s, Granlund & Montgomery, PLDI 1994).
+struct FastDiv {
+    uint32_t d;
+    uint32_t m;
+    uint32_t s;
+    uint32_t pad_;
+};
+
+enum IoMode : int32_t {
+    IO_C2C = 0,        // complex in, complex out
+    IO_R2C_EVEN = 1,   // real N = 2L in (read as L complex), L-point FFT, split post-process to L+1 bins
+    IO_C2R_EVEN = 2,   // L+1 bins in, pre-process to L complex, L-point FFT, 2L reals out
+    IO_R2C_FULL = 3,   // real L in, L-point FFT, L/2+1 bins out (odd lengths)
+    IO_C2R_FULL = 4,   // L/2+1 bins in, Hermitian extension to L, L-point FFT, L reals out (odd)
+};
+
+struct TileDesc {
+    int64_t n0, n1;    // f = f0 + n0 * (f1 + n1 * f2)
+    int64_t s0, s1, s2;  // strides of f0, f1, f2
+    int64_t es;        // stride between elements of one transform
+    int32_t unit_f;    // 1 when consecutive transforms are adjacent (column pass): walk f fastest
+    int32_t vec2;      // 1 when es == 1 and every row start is 16-byte aligned (float4 accesses)
+    FastDiv div_n0, div_n1;
+};
+
+struct StockhamArgs {
+    const void *in;
+    void *out;
+    const float2 *tw;      // pass twiddles (FP64-generated, rounded to FP32), layout fixed by the radix sequence
+    const float2 *rtw;     // real-transform roots w^k = e^{s 2 pi i k/(2L)}, k = 0..L
+    const float2 *tw4_lo;  // four-step twiddles W^m = lo[m & mask] * hi[m >> shift], W = e^{s 2 pi i/Ntot}
+    const float2 *tw4_hi;
+    int64_t nfft;          // number of transforms in this launch (< 2^31)
+    int64_t tw4_ntot;      // 0 = no four-step twiddle
+    int64_t tw4_mask;
+    int32_t tw4_shift;
+    int32_t F;             // transforms per workgroup
+    int32_t LP;            // LDS pitch (floats) of one transform in a plane
+    int32_t io_mode;
+    int32_t in_len;        // elements per transform on input (L, L+1 bins, L reals ...)
+    int32_t out_len;       // elements per transform on output
+    FastDiv div_in_len;    // divides by in_len
+    FastDiv div_out_len;   // divides by out_len
+    FastDiv div_F;         // divides by F
+    FastDiv div_col;       // four-step column index = f0(in) / col_div
+    FastDiv div_in_vec;    // divides by in_len / 2 (16-byte path)
+    FastDiv div_out_vec;   // divides by out_len / 2
+    TileDesc tin, tout;
+};
+
+// Host-side view of one compiled kernel instance.
+typedef void (*StockhamKernelFn)(StockhamArgs);
+struct StockhamInstance {
+    int32_t L;
+    int32_t T;               // threads per transform
+    int32_t npass;
+    int32_t radix[12];
+    const void *fwd;         // __global__ function handles (hipLaunchKernel symbol)
+    const void *inv;
+};
+
+}  // namespace sip

@@ -1,0 +1,320 @@
+// LDS-staged Stockham FFT kernels for gfx950 (the plain method).
+//
+// A workgroup owns F transforms of length L (a "tile"). The tile is read from HBM with the
+// unit-stride axis walked by consecutive lanes (rows for contiguous batches, transforms for
+// column passes), kept in LDS as separate re / im float planes padded by one word every 16
+// (so both row-wise and column-wise lane patterns avoid bank conflicts), pushed through the
+// radix passes, and written back the same way. Each pass is the Stockham autosort step:
+// butterfly b reads elements b + r*L/R, multiplies by W_{ns R}^{(b mod ns) r}, runs an R-point DFT
+// in registers and writes elements (b/ns)*ns*R + (b mod ns) + q*ns. The radix sequence, the
+// thread count per transform and therefore every LDS index are compile-time constants of the
+// kernel instance; only the tile addressing (strides, batch) is runtime.
+// Twiddles come from FP64-generated tables rounded once to FP32.
+//
+// The reference computes the same recursion as staged DFT-matrix GEMMs on the Ascend cube unit
+// (FFT-B four-step, ops/fft/c2c/fft_b/fft_b/kernel/fft_b.cce:2290-2576; FFT-Mix multi-step,
+// ops/fft/common/include/kernel/fft_all_common.cceh:593-616); here the small DFTs run as VALU
+// butterflies and the re-layouts between stages stay in LDS.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "butterflies.h"
+#include "stockham_args.h"
+
+namespace sip {
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &d)
+{
+    return (__umulhi(n, d.m) + n) >> d.s;
+}
+
+__device__ __forceinline__ int lds_pad(int j) { return j + (j >> 4); }
+
+// One radix-R pass over a transform of length L held in LDS, T threads per transform,
+// NS = product of the radices of the earlier passes, TW = this pass's twiddles [k][r-1].
+template <int L, int T, int S, int NS, int R>
+__device__ __forceinline__ void pass_static(float *re, float *im, const float2 *__restrict__ tw, int lbase, int t,
+                                            bool active)
+{
+    constexpr int NBF = L / R;                 // butterflies per transform
+    constexpr int NB = (NBF + T - 1) / T;      // butterflies per thread
+    constexpr bool EXACT = (NBF % T) == 0;
+    float2 v[NB][R];
+    float2 w[NB][R];
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+        const int b = t + q * T;
+        if (active && (EXACT || b < NBF)) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int j = lds_pad(b + r * NBF);
+                v[q][r] = make_float2(re[lbase + j], im[lbase + j]);
+            }
+            if constexpr (NS > 1) {
+                const int k = b % NS;
+                const float2 *tp = tw + k * (R - 1);
+#pragma unroll
+                for (int r = 1; r < R; r++) w[q][r] = tp[r - 1];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+        const int b = t + q * T;
+        if (active && (EXACT || b < NBF)) {
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) v[q][r] = cmul(v[q][r], w[q][r]);
+            }
+            dft<R, S>(v[q]);
+            const int k = b % NS;
+            const int d = (b / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int j = lds_pad(d + r * NS);
+                re[lbase + j] = v[q][r].x;
+                im[lbase + j] = v[q][r].y;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <int L, int T, int S, int NS, int R, int... Rest>
+__device__ __forceinline__ void passes_static(float *re, float *im, const float2 *__restrict__ tw, int lbase, int t,
+                                              bool active)
+{
+    pass_static<L, T, S, NS, R>(re, im, tw, lbase, t, active);
+    if constexpr (sizeof...(Rest) > 0)
+        passes_static<L, T, S, NS * R, Rest...>(re, im, tw + NS * (R - 1), lbase, t, active);
+}
+
+// f = f0 + n0 * (f1 + n1 * f2)
+__device__ __forceinline__ void tile_split(uint32_t f, const TileDesc &d, uint32_t &f0, uint32_t &f1, uint32_t &f2)
+{
+    const uint32_t g = fdiv(f, d.div_n0);
+    f0 = f - g * (uint32_t)d.n0;
+    f2 = fdiv(g, d.div_n1);
+    f1 = g - f2 * (uint32_t)d.n1;
+}
+
+// W^m with W = e^{s 2 pi i / Ntot}, from the two-level table (m < Ntot).
+__device__ __forceinline__ float2 tw4_root(const StockhamArgs &a, int64_t m)
+{
+    return cmul(a.tw4_lo[m & a.tw4_mask], a.tw4_hi[m >> a.tw4_shift]);
+}
+
+template <int L, int T, int S, int... Rs>
+__global__ __launch_bounds__(1024) void stockham_kernel(const StockhamArgs a)
+{
+    extern __shared__ float lds[];
+    const int nthreads = blockDim.x;
+    const int tid = threadIdx.x;
+    const int F = a.F;
+    const int LP = a.LP;
+    float *re = lds;
+    float *im = lds + F * LP;
+    int64_t *in_base = reinterpret_cast<int64_t *>(lds + 2 * F * LP);
+    int64_t *out_base = in_base + F;
+    int64_t *col = out_base + F;
+
+    const int64_t fstart = (int64_t)blockIdx.x * F;
+    const int nf = (int)((a.nfft - fstart) < F ? (a.nfft - fstart) : F);
+
+    // Per-transform base offsets (and the four-step column index).
+    for (int fl = tid; fl < nf; fl += nthreads) {
+        const uint32_t f = (uint32_t)(fstart + fl);
+        uint32_t f0, f1, f2;
+        tile_split(f, a.tin, f0, f1, f2);
+        in_base[fl] = (int64_t)f0 * a.tin.s0 + (int64_t)f1 * a.tin.s1 + (int64_t)f2 * a.tin.s2;
+        col[fl] = fdiv(f0, a.div_col);
+        tile_split(f, a.tout, f0, f1, f2);
+        out_base[fl] = (int64_t)f0 * a.tout.s0 + (int64_t)f1 * a.tout.s1 + (int64_t)f2 * a.tout.s2;
+    }
+    __syncthreads();
+
+    // ---------------------------------------------------------------- load tile
+    const int in_len = a.in_len;
+    if (a.io_mode == IO_R2C_FULL) {
+        const float *src = static_cast<const float *>(a.in);
+        const int total = F * in_len;
+        for (int i = tid; i < total; i += nthreads) {
+            int fl, j;
+            if (a.tin.unit_f) {
+                j = (int)fdiv((uint32_t)i, a.div_F);
+                fl = i - j * F;
+            } else {
+                fl = (int)fdiv((uint32_t)i, a.div_in_len);
+                j = i - fl * in_len;
+            }
+            if (fl < nf) {
+                re[fl * LP + lds_pad(j)] = src[in_base[fl] + (int64_t)j * a.tin.es];
+                im[fl * LP + lds_pad(j)] = 0.f;
+            }
+        }
+    } else if (a.tin.vec2) {
+        const float4 *src = static_cast<const float4 *>(a.in);
+        const int half = in_len >> 1;
+        const int total = F * half;
+        for (int i = tid; i < total; i += nthreads) {
+            const int fl = (int)fdiv((uint32_t)i, a.div_in_vec);
+            const int jj = i - fl * half;
+            if (fl < nf) {
+                const float4 v = src[(in_base[fl] >> 1) + jj];
+                const int j = 2 * jj;
+                re[fl * LP + lds_pad(j)] = v.x;
+                im[fl * LP + lds_pad(j)] = v.y;
+                re[fl * LP + lds_pad(j + 1)] = v.z;
+                im[fl * LP + lds_pad(j + 1)] = v.w;
+            }
+        }
+    } else {
+        const float2 *src = static_cast<const float2 *>(a.in);
+        const int total = F * in_len;
+        for (int i = tid; i < total; i += nthreads) {
+            int fl, j;
+            if (a.tin.unit_f) {
+                j = (int)fdiv((uint32_t)i, a.div_F);
+                fl = i - j * F;
+            } else {
+                fl = (int)fdiv((uint32_t)i, a.div_in_len);
+                j = i - fl * in_len;
+            }
+            if (fl < nf) {
+                const float2 v = src[in_base[fl] + (int64_t)j * a.tin.es];
+                re[fl * LP + lds_pad(j)] = v.x;
+                im[fl * LP + lds_pad(j)] = v.y;
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---------------------------------------------------------------- pre-process (C2R)
+    if (a.io_mode == IO_C2R_EVEN) {
+        // P[k] = (X[k] + conj X[L-k]) + i w^k (X[k] - conj X[L-k]); imag of X[0] and X[L] dropped.
+        constexpr int NPAIR = L / 2 + 1;
+        for (int i = tid; i < F * NPAIR; i += nthreads) {
+            const int fl = i / NPAIR;
+            const int k = i - fl * NPAIR;
+            if (fl >= nf) continue;
+            float *pr = re + fl * LP, *pi = im + fl * LP;
+            const int k2 = L - k;
+            float2 xk = make_float2(pr[lds_pad(k)], pi[lds_pad(k)]);
+            float2 xm = make_float2(pr[lds_pad(k2)], pi[lds_pad(k2)]);
+            if (k == 0) {
+                xk.y = 0.f;
+                xm.y = 0.f;
+            }
+            {
+                const float2 c = cconj(xm);
+                const float2 s = cadd(xk, c), d = csub(xk, c);
+                const float2 p = cadd(s, mul_si<1>(cmul(a.rtw[k], d)));
+                pr[lds_pad(k)] = p.x;
+                pi[lds_pad(k)] = p.y;
+            }
+            if (k != 0 && k2 != k) {
+                const float2 c = cconj(xk);
+                const float2 s = cadd(xm, c), d = csub(xm, c);
+                const float2 p = cadd(s, mul_si<1>(cmul(a.rtw[k2], d)));
+                pr[lds_pad(k2)] = p.x;
+                pi[lds_pad(k2)] = p.y;
+            }
+        }
+        __syncthreads();
+    } else if (a.io_mode == IO_C2R_FULL) {
+        // Hermitian extension of bins 0..L/2 to L points; imag of bin 0 (and L/2) dropped.
+        constexpr int H = L / 2;
+        for (int i = tid; i < F * L; i += nthreads) {
+            const int fl = i / L;
+            const int j = i - fl * L;
+            if (fl >= nf) continue;
+            float *pr = re + fl * LP, *pi = im + fl * LP;
+            if (j > H) {
+                pr[lds_pad(j)] = pr[lds_pad(L - j)];
+                pi[lds_pad(j)] = -pi[lds_pad(L - j)];
+            }
+        }
+        __syncthreads();
+        for (int fl = tid; fl < nf; fl += nthreads) {
+            im[fl * LP] = 0.f;
+            if ((L & 1) == 0) im[fl * LP + lds_pad(H)] = 0.f;
+        }
+        __syncthreads();
+    }
+
+    // ---------------------------------------------------------------- radix passes
+    {
+        const int flp = tid / T;
+        const int t = tid - flp * T;
+        const bool active = flp < nf;
+        const int lbase = (active ? flp : 0) * LP;
+        passes_static<L, T, S, 1, Rs...>(re, im, a.tw, lbase, t, active);
+    }
+
+    // ---------------------------------------------------------------- store tile
+    const int out_len = a.out_len;
+    if (a.io_mode == IO_R2C_EVEN) {
+        // X[k] = Z[k] (1 - i w^k)/2 + conj Z[L-k] (1 + i w^k)/2, k = 0..L
+        float2 *dst = static_cast<float2 *>(a.out);
+        for (int i = tid; i < F * out_len; i += nthreads) {
+            const int fl = (int)fdiv((uint32_t)i, a.div_out_len);
+            const int k = i - fl * out_len;
+            if (fl >= nf) continue;
+            const float *pr = re + fl * LP, *pi = im + fl * LP;
+            const int ka = (k == L) ? 0 : k;
+            const int kb = (k == 0) ? 0 : L - k;
+            const float2 z = make_float2(pr[lds_pad(ka)], pi[lds_pad(ka)]);
+            const float2 zc = make_float2(pr[lds_pad(kb)], -pi[lds_pad(kb)]);
+            const float2 iw = mul_si<1>(a.rtw[k]);
+            const float2 A = make_float2(0.5f * (1.f - iw.x), -0.5f * iw.y);
+            const float2 B = make_float2(0.5f * (1.f + iw.x), 0.5f * iw.y);
+            dst[out_base[fl] + (int64_t)k * a.tout.es] = cadd(cmul(z, A), cmul(zc, B));
+        }
+    } else if (a.io_mode == IO_C2R_FULL) {
+        float *dst = static_cast<float *>(a.out);
+        for (int i = tid; i < F * out_len; i += nthreads) {
+            int fl, j;
+            if (a.tout.unit_f) {
+                j = (int)fdiv((uint32_t)i, a.div_F);
+                fl = i - j * F;
+            } else {
+                fl = (int)fdiv((uint32_t)i, a.div_out_len);
+                j = i - fl * out_len;
+            }
+            if (fl < nf) dst[out_base[fl] + (int64_t)j * a.tout.es] = re[fl * LP + lds_pad(j)];
+        }
+    } else if (a.tout.vec2 && a.tw4_ntot == 0) {
+        float4 *dst = static_cast<float4 *>(a.out);
+        const int half = out_len >> 1;
+        for (int i = tid; i < F * half; i += nthreads) {
+            const int fl = (int)fdiv((uint32_t)i, a.div_out_vec);
+            const int jj = i - fl * half;
+            if (fl < nf) {
+                const int j = 2 * jj;
+                const float *pr = re + fl * LP, *pi = im + fl * LP;
+                dst[(out_base[fl] >> 1) + jj] =
+                    make_float4(pr[lds_pad(j)], pi[lds_pad(j)], pr[lds_pad(j + 1)], pi[lds_pad(j + 1)]);
+            }
+        }
+    } else {
+        float2 *dst = static_cast<float2 *>(a.out);
+        for (int i = tid; i < F * out_len; i += nthreads) {
+            int fl, j;
+            if (a.tout.unit_f) {
+                j = (int)fdiv((uint32_t)i, a.div_F);
+                fl = i - j * F;
+            } else {
+                fl = (int)fdiv((uint32_t)i, a.div_out_len);
+                j = i - fl * out_len;
+            }
+            if (fl < nf) {
+                float2 v = make_float2(re[fl * LP + lds_pad(j)], im[fl * LP + lds_pad(j)]);
+                if (a.tw4_ntot != 0) v = cmul(v, tw4_root(a, col[fl] * (int64_t)j));
+                dst[out_base[fl] + (int64_t)j * a.tout.es] = v;
+            }
+        }
+    }
+}
+
+}  // namespace sip

@@ -1,0 +1,161 @@
+"""GPU parity of the plain form (Stockham kernels through the C-ABI) against the FP64 oracle.
+
+Tolerances (rel-L2 vs FP64): 1e-6 for C2C N=1024 b=256, 2e-6 for every other plain transform,
+as stated for this path; Bluestein lengths (prime factors > 13) 5e-6. Round trips
+inverse(forward(x)) / N are checked against x at 2e-6.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from fftcase import FWD, INV, c2c, c2r, r2c, report, sign_of
+
+pytestmark = pytest.mark.gpu
+
+TOL = 2e-6
+
+
+def _c2c_case(n, batch, direction, seed, tol=TOL):
+    x = oracle.uniform_complex((batch, n), seed)
+    y = c2c(x, [n], direction)
+    ref = oracle.c2c(x, sign_of(direction))
+    rel, _ = report(f"c2c n={n} b={batch} dir={direction:#x}", y, ref)
+    assert rel <= tol
+    return x, y
+
+
+@pytest.mark.parametrize("direction", [FWD, INV])
+def test_c1_c2c_1024_b256(torch_cuda, direction):
+    _c2c_case(1024, 256, direction, seed=0x5EED0001, tol=1e-6)
+
+
+def test_c2_c2c_4096_b16384_full(torch_cuda):
+    """Headline workload at full size: every output bin against the FP64 oracle, and the round trip."""
+    n, b = 4096, 16384
+    x, y = _c2c_case(n, b, FWD, seed=0x5EED0002)
+    back = c2c(y, [n], INV) / n
+    rel, _ = report("roundtrip 4096x16384", back, x.astype(np.complex128))
+    assert rel <= TOL
+
+
+@pytest.mark.parametrize("n", [1536, 2000])
+def test_c2_mixed_radix_b8192(torch_cuda, n):
+    x, y = _c2c_case(n, 8192, FWD, seed=0x5EED0003 + n)
+    back = c2c(y, [n], INV) / n
+    rel, _ = report(f"roundtrip {n}x8192", back, x.astype(np.complex128))
+    assert rel <= TOL
+
+
+def test_c2_r2c_c2r_8192_b4096(torch_cuda):
+    n, b = 8192, 4096
+    x = oracle.uniform(n * b, 0x5EED0004).reshape(b, n)
+    X = r2c(x, [n])
+    rel, _ = report("r2c 8192x4096", X, oracle.r2c(x))
+    assert rel <= TOL
+    back = c2r(X, [n]) / n
+    rel, _ = report("c2r roundtrip 8192x4096", back, x.astype(np.float64))
+    assert rel <= TOL
+    # C2R against the oracle on the GPU's own R2C output
+    y = c2r(X, [n])
+    rel, _ = report("c2r 8192x4096", y, oracle.c2r(X, n))
+    assert rel <= TOL
+
+
+POW2 = [2 ** k for k in range(1, 15)]
+SMOOTH = [3, 5, 6, 9, 10, 12, 15, 25, 27, 45, 60, 81, 100, 125, 243, 360, 625, 768, 1000, 1080, 2187, 2400,
+          3000, 3125, 3840, 4000]
+PRIME_FACTOR = [7, 11, 13, 14, 21, 49, 77, 91, 143, 169, 343, 1001, 1331, 2197, 4004]
+
+
+@pytest.mark.parametrize("n", POW2 + SMOOTH + PRIME_FACTOR)
+def test_c2c_leaf_lengths(torch_cuda, n):
+    batch = max(1, min(64, (1 << 18) // n))
+    for direction in (FWD, INV):
+        _c2c_case(n, batch, direction, seed=1000 + n)
+
+
+@pytest.mark.parametrize("n", [1, 17, 31, 97, 127, 997, 199 * 199, 6007])
+def test_c2c_bluestein_and_trivial(torch_cuda, n):
+    batch = 3
+    _c2c_case(n, batch, FWD, seed=2000 + n, tol=5e-6)
+    _c2c_case(n, batch, INV, seed=3000 + n, tol=5e-6)
+
+
+@pytest.mark.parametrize("n", [32768, 65536, 3 ** 10, 5 ** 7, 2 ** 20, 15000, 2 ** 17 * 3])
+def test_c2c_four_step(torch_cuda, n):
+    batch = max(1, (1 << 21) // n)
+    x, y = _c2c_case(n, batch, FWD, seed=4000 + n)
+    back = c2c(y, [n], INV) / n
+    rel, _ = report(f"roundtrip {n}", back, x.astype(np.complex128))
+    assert rel <= TOL
+
+
+def test_c2c_2_pow_27(torch_cuda):
+    """Largest size the reference accepts (0 < N <= 2^27): three-level four-step."""
+    n = 1 << 27
+    x = oracle.uniform_complex((1, n), 77)
+    y = c2c(x, [n], FWD)
+    ref = oracle.c2c(x, oracle.FORWARD)
+    rel, _ = report("c2c 2^27", y, ref)
+    assert rel <= TOL
+
+
+@pytest.mark.parametrize("n", [4, 6, 8, 10, 16, 126, 128, 256, 500, 1024, 2000, 4096, 8192, 16384, 32768,
+                               65536, 3, 5, 7, 9, 15, 243, 567, 997, 2187, 199 * 199, 15000])
+def test_r2c_c2r_lengths(torch_cuda, n):
+    b = max(1, min(32, (1 << 19) // n))
+    x = oracle.uniform(n * b, 5000 + n).reshape(b, n)
+    tol = 5e-6 if n in (997, 199 * 199) else TOL
+    X = r2c(x, [n])
+    rel, _ = report(f"r2c n={n}", X, oracle.r2c(x))
+    assert rel <= tol
+    Xh = oracle.r2c(x).astype(np.complex64)
+    y = c2r(Xh, [n])
+    rel, _ = report(f"c2r n={n}", y, oracle.c2r(Xh, n))
+    assert rel <= tol
+
+
+def test_reference_example_input(torch_cuda):
+    """The reference example's input x[i] = (i, i+1) over batch 32 x N 128
+    (example/A2/FFT/1d/example_acl_fft_c2c_1d.cpp:99-110), against the committed golden vectors."""
+    from golden import load_fixture
+    fx = load_fixture("example_c2c_128x32")
+    y = c2c(fx["input"], [128], FWD)
+    rel, _ = report("example c2c 128x32", y, fx["output"])
+    assert rel <= 1e-6
+    fx = load_fixture("example_r2c_256x32")
+    y = r2c(fx["input"], [256], FWD)
+    rel, _ = report("example r2c 256x32", y, fx["output"])
+    assert rel <= 1e-6
+    fx = load_fixture("example_c2r_128x32_forward")
+    y = c2r(fx["input"], [128], FWD)
+    rel, _ = report("example c2r 128x32 forward", y, fx["output"])
+    assert rel <= 1e-6
+
+
+def test_direction_quirks(torch_cuda):
+    """R2C with INVERSE computes the e^{+} transform; C2R with FORWARD the e^{-} transform of the
+    Hermitian extension (dft_r2c_core.cpp:96-101, dft_c2r_core.cpp:90-105)."""
+    n, b = 1000, 4
+    x = oracle.uniform(n * b, 99).reshape(b, n)
+    X = r2c(x, [n], INV)
+    rel, _ = report("r2c inverse", X, oracle.r2c(x, oracle.INVERSE))
+    assert rel <= TOL
+    Xh = oracle.r2c(x).astype(np.complex64)
+    y = c2r(Xh, [n], FWD)
+    rel, _ = report("c2r forward", y, oracle.c2r(Xh, n, oracle.FORWARD))
+    assert rel <= TOL
+
+
+def test_inplace_c2c(torch_cuda):
+    import sip_amd as S
+    from fftcase import make, to_dev
+    n, b = 4096, 64
+    x = oracle.uniform_complex((b, n), 4242)
+    h = make([n], S.ASCEND_FFT_C2C, FWD, b)
+    xd = to_dev(x)
+    assert h.exec_c2c(xd, xd) == 0
+    assert h.synchronize() == 0
+    rel, _ = report("in-place 4096", xd.cpu().numpy(), oracle.c2c(x))
+    h.destroy()
+    assert rel <= TOL
