@@ -1,0 +1,206 @@
+"""Benchmark of the FFT hot path on one MI355X.
+
+Workload (the metric's headline configuration): batched 1D C2C complex64 forward FFT, N = 4096,
+batch = 16384, inputs uniform[-1,1) already resident in HBM, one asdFftExecC2C per step (one kernel
+launch). value = 5 N log2 N * batch * steps / wall time (GFLOP/s), timed over `--steps` steps
+bracketed by device synchronisation after `--warmup` untimed steps.
+
+roofline: algorithmic bytes per launch (one read + one write of 16384 x 4096 complex64 = 1 GiB)
+divided by the kernel's average duration, measured with HIP events recorded on the plan's stream
+around the timed steps; peak = 8 TB/s HBM. traffic (HBM bytes per launch from rocprofv3 PMC
+FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel's reading) is read from
+profiles/traffic_c2c_4096x16384.json when present.
+
+cpu_baseline: the oracle's FP32 CPU FFT (same staged Stockham algorithm) on the host cores, median of
+runs over the full 16384 x 4096 batch; the leg also checks a sample of the GPU's output rows against
+the FP64 oracle. Secondary configurations (R2C/C2R 8192 x 4096, C2C 1536 / 2000 x 8192, C2C 1024 x 256)
+are timed the same way and reported under "secondary".
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-secondary", action="store_true")
+    return ap.parse_args()
+
+
+def timed_exec(torch, h, kind, x, y, steps, warmup, stream):
+    fn = {"c2c": h.exec_c2c, "r2c": h.exec_r2c, "c2r": h.exec_c2r}[kind]
+    with torch.cuda.stream(stream):
+        for _ in range(warmup):
+            assert fn(x, y) == 0
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(steps):
+            assert fn(x, y) == 0
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    dev_ms = ev0.elapsed_time(ev1)
+    return wall, dev_ms / steps
+
+
+def make_handle(S, torch, dims, fft_type, batch, stream):
+    h = S.FftHandle()
+    if len(dims) == 1:
+        st = h.make_plan_1d(dims[0], fft_type, S.ASCEND_FFT_FORWARD, batch)
+    else:
+        st = h.make_plan_2d(dims[0], dims[1], fft_type, S.ASCEND_FFT_FORWARD, batch)
+    assert st == 0, st
+    h.alloc_workspace()
+    assert h.set_stream(stream.cuda_stream) == 0
+    return h
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the batched 1D path shards "
+                                   "by splitting the batch, see DESIGN.md)"}))
+        sys.exit(2)
+    import torch
+    import sip_amd as S
+
+    torch.cuda.set_device(0)
+    stream = torch.cuda.Stream()
+    n, batch = 4096, 16384
+    flops = 5.0 * n * math.log2(n) * batch
+    alg_bytes = 16.0 * n * batch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0x5EED0002)
+    with torch.cuda.stream(stream):
+        x = (torch.rand((batch, n, 2), generator=g, device="cuda", dtype=torch.float32) * 2 - 1)
+        x = torch.view_as_complex(x).contiguous()
+        y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    h = make_handle(S, torch, [n], S.ASCEND_FFT_C2C, batch, stream)
+    wall, kern_ms = timed_exec(torch, h, "c2c", x, y, args.steps, args.warmup, stream)
+    value = flops * args.steps / wall / 1e9
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_c2c_4096x16384.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+    # device-to-device copy of the same 512 MiB (read + write = the FFT's algorithmic bytes): the
+    # practical HBM ceiling next to the 8 TB/s peak
+    with torch.cuda.stream(stream):
+        z = torch.empty_like(x)
+        for _ in range(3):
+            z.copy_(x)
+        c0 = torch.cuda.Event(enable_timing=True)
+        c1 = torch.cuda.Event(enable_timing=True)
+        c0.record(stream)
+        for _ in range(10):
+            z.copy_(x)
+        c1.record(stream)
+    torch.cuda.synchronize()
+    copy_gbs = alg_bytes / (c0.elapsed_time(c1) / 10 * 1e-3) / 1e9
+    del z
+
+    out = {
+        "metric": "FFT GFLOP/s (5N log2N) + %HBM roofline at rel-L2<=2e-6; serving p50/p99 ms",
+        "value": round(value, 1),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (uniform[-1,1) complex64, seeded torch generator)",
+        "config": {"workload": "plain batched 1D C2C forward, complex64, N=4096, batch=16384",
+                   "n": n, "batch": batch, "direction": "forward"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "kernel_ms": round(kern_ms, 5), "algorithmic_bytes": int(alg_bytes),
+                     "copy_GB/s": round(copy_gbs, 1)},
+    }
+
+    if not args.no_secondary:
+        sec = {}
+        cases = [("c2c_1024x256", [1024], S.ASCEND_FFT_C2C, 256),
+                 ("c2c_1536x8192", [1536], S.ASCEND_FFT_C2C, 8192),
+                 ("c2c_2000x8192", [2000], S.ASCEND_FFT_C2C, 8192),
+                 ("r2c_8192x4096", [8192], S.ASCEND_FFT_R2C, 4096),
+                 ("c2r_8192x4096", [8192], S.ASCEND_FFT_C2R, 4096),
+                 ("c2c_2d_1024x1024x8", [1024, 1024], S.ASCEND_FFT_C2C, 8)]
+        for name, dims, ft, b in cases:
+            nn = int(math.prod(dims))
+            with torch.cuda.stream(stream):
+                if ft == S.ASCEND_FFT_R2C:
+                    xi = torch.rand((b, nn), device="cuda") * 2 - 1
+                    yo = torch.empty((b, nn // 2 + 1), dtype=torch.complex64, device="cuda")
+                    kind, fl, by = "r2c", 2.5 * nn * math.log2(nn) * b, 4.0 * nn * b + 8.0 * (nn // 2 + 1) * b
+                elif ft == S.ASCEND_FFT_C2R:
+                    xi = torch.view_as_complex(torch.rand((b, nn // 2 + 1, 2), device="cuda")).contiguous()
+                    yo = torch.empty((b, nn), dtype=torch.float32, device="cuda")
+                    kind, fl, by = "c2r", 2.5 * nn * math.log2(nn) * b, 4.0 * nn * b + 8.0 * (nn // 2 + 1) * b
+                else:
+                    shape = [b] + dims
+                    xi = torch.view_as_complex(torch.rand(shape + [2], device="cuda") * 2 - 1).contiguous()
+                    yo = torch.empty_like(xi)
+                    kind, fl, by = "c2c", 5.0 * nn * math.log2(nn) * b, 16.0 * nn * b
+            torch.cuda.synchronize()
+            hh = make_handle(S, torch, dims, ft, b, stream)
+            st = max(10, args.steps // 2)
+            w, kms = timed_exec(torch, hh, kind, xi, yo, st, 3, stream)
+            sec[name] = {"GFLOP/s": round(fl * st / w / 1e9, 1), "ms_per_exec": round(w / st * 1e3, 4),
+                         "device_ms": round(kms, 5), "hbm_GB/s": round(by / (kms * 1e-3) / 1e9, 1),
+                         "hbm_frac": round(by / (kms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                         "flop_convention": "2.5N log2N" if kind != "c2c" else "5N log2N"}
+            hh.destroy()
+            del xi, yo
+        out["secondary"] = sec
+
+    if not args.no_cpu:
+        import numpy as np
+        import oracle
+        cores = len(os.sched_getaffinity(0))
+        threads = min(cores, 16)
+        xs = x.cpu().numpy()
+        ys = y.cpu().numpy()
+        oracle.c2c_f32(xs[:256], threads=threads)  # warm-up
+        times = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            oracle.c2c_f32(xs, threads=threads)
+            times.append(time.perf_counter() - t0)
+        tmed = sorted(times)[len(times) // 2]
+        rows = np.arange(0, batch, batch // 64)
+        rel, mx = oracle.error(ys[rows], oracle.c2c(xs[rows]))
+        out["cpu_baseline"] = {"value": round(flops / tmed / 1e9, 2), "unit": "GFLOP/s", "cores": threads,
+                               "kind": "port",
+                               "sample": "full C2C N=4096 x batch 16384, FP32 staged Stockham (oracle/fft_oracle.c), "
+                                         "OpenMP over the batch, median of 5 runs",
+                               "seconds_per_batch": round(tmed, 3),
+                               "gpu_rel_l2_vs_fp64_sample": rel, "gpu_max_abs_vs_fp64_sample": mx,
+                               "gpu_check_rows": int(rows.size)}
+    h.destroy()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

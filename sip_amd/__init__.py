@@ -14,7 +14,7 @@ import json
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsip_fft.so")
+LIB_PATH = os.environ.get("SIP_FFT_LIB") or os.path.join(HERE, "libsip_fft.so")
 
 # enums (reference include/fft_api.h:26-44)
 ASCEND_FFT_C2C = 0x10

@@ -66,9 +66,15 @@ def _compile(job):
     return src, None
 
 
-def build(force: bool = True, jobs: int | None = None, verbose: bool = False) -> str:
+def build(force: bool = True, jobs: int | None = None, verbose: bool = False, extra_device_flags=(),
+          out: str | None = None, obj_dir: str | None = None) -> str:
+    """Compiles and links the library. extra_device_flags / out / obj_dir build a variant (for
+    tuning experiments) next to the default one."""
+    out = out or LIB
+    obj_root = obj_dir or OBJ
     os.makedirs(GEN, exist_ok=True)
-    os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(obj_root, exist_ok=True)
+    dflags = DEVICE_FLAGS + list(extra_device_flags)
     subprocess.run([sys.executable, os.path.join(CSRC, "kernels", "gen_instances.py"), GEN, str(NPARTS)],
                    check=True, capture_output=not verbose)
     headers = _headers()
@@ -76,11 +82,11 @@ def build(force: bool = True, jobs: int | None = None, verbose: bool = False) ->
     dev_srcs = sorted(glob.glob(os.path.join(CSRC, "kernels", "*.hip"))) + sorted(glob.glob(os.path.join(GEN, "*.hip")))
     host_srcs = sorted(glob.glob(os.path.join(CSRC, "host", "*.cpp")))
     for src in dev_srcs:
-        obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-        cmd = [HIPCC] + DEVICE_FLAGS + ["-c", src, "-o", obj]
-        work.append((src, obj, cmd, _digest(src, DEVICE_FLAGS, headers), force))
+        obj = os.path.join(obj_root, os.path.basename(src) + ".o")
+        cmd = [HIPCC] + dflags + ["-c", src, "-o", obj]
+        work.append((src, obj, cmd, _digest(src, dflags, headers), force))
     for src in host_srcs:
-        obj = os.path.join(OBJ, os.path.basename(src) + ".o")
+        obj = os.path.join(obj_root, os.path.basename(src) + ".o")
         cmd = [CLANGXX] + HOST_FLAGS + ["-c", src, "-o", obj]
         work.append((src, obj, cmd, _digest(src, HOST_FLAGS, headers), force))
     jobs = jobs or min(8, os.cpu_count() or 4)
@@ -93,13 +99,13 @@ def build(force: bool = True, jobs: int | None = None, verbose: bool = False) ->
         msg = "\n".join(f"--- {s}\n{e}" for s, e in errors)
         raise RuntimeError("libsip_fft build failed:\n" + msg)
     objs = [w[1] for w in work]
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr[-4000:])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

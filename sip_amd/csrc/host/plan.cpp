@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <unordered_map>
@@ -282,6 +283,13 @@ AuxRows to_rows(const Lay &l, bool *ok)
     return r;
 }
 
+bool env_flag(const char *name, bool dflt)
+{
+    const char *e = std::getenv(name);
+    if (!e || !e[0]) return dflt;
+    return e[0] != '0';
+}
+
 constexpr int64_t kColumnLeafMax = 1024;
 constexpr int64_t kRowLeafMax = 16384;
 
@@ -356,6 +364,10 @@ struct Builder {
         } else {
             F = std::max(1, 256 / T);
             while (F > 1 && ((size_t)F * per > 48 * 1024 || F * T > 1024)) F--;
+            if (const char *e = std::getenv("SIP_FFT_ROW_F")) {
+                const int f = std::atoi(e);
+                if (f >= 1 && f * T <= 1024) F = f;
+            }
         }
         if (F > nfft) F = (int)nfft;
         if (F < 1) F = 1;
@@ -370,6 +382,17 @@ struct Builder {
         if (out_len % 2) a.tout.vec2 = 0;
         if (io_mode == IO_R2C_FULL) a.tin.vec2 = 0;
         if (io_mode == IO_C2R_FULL) a.tout.vec2 = 0;
+        // First / last radix pass straight from / to HBM for contiguous complex rows, when a
+        // transform has enough threads for its accesses to coalesce.
+        a.direct_in = (io_mode == IO_C2C || io_mode == IO_R2C_EVEN) && a.tin.es == 1 && !a.tin.unit_f && T >= 16;
+        a.direct_out = (io_mode == IO_C2C || io_mode == IO_C2R_EVEN) && a.tout.es == 1 && !a.tout.unit_f &&
+                       T >= 16 && tw4_ntot == 0;
+        // Measured on MI355X (N=4096 x 16384): the two-plane kernel in launch order beats the
+        // single-plane exchange kernel (twice the resident workgroups) and XCD-contiguous tile order;
+        // both stay available as tuning switches SIP_FFT_SPLIT_LDS=1 and SIP_FFT_XCD=1.
+        const bool rows = io_mode == IO_C2C && a.direct_in && a.direct_out && inst->rows_fwd != nullptr;
+        a.split_lds = rows && env_flag("SIP_FFT_SPLIT_LDS", false);
+        a.xcd_remap = env_flag("SIP_FFT_XCD", false);
         st.tw = pass_twiddles(inst, sign);
         if (io_mode == IO_R2C_EVEN || io_mode == IO_C2R_EVEN) st.rtw = real_roots(L, sign);
         if (tw4_ntot) {
@@ -381,7 +404,13 @@ struct Builder {
         }
         st.grid = dim3((unsigned)((nfft + F - 1) / F));
         st.block = dim3((unsigned)(F * T));
-        st.lds = (size_t)2 * F * lp * sizeof(float) + (size_t)3 * F * sizeof(int64_t);
+        st.lds = a.split_lds ? (size_t)F * lp * sizeof(float)
+                             : (size_t)2 * F * lp * sizeof(float) + (size_t)3 * F * sizeof(int64_t);
+        if (const char *e = std::getenv("SIP_FFT_LDS_MIN_KB")) {
+            // occupancy throttle for tuning: allocate at least this much LDS per workgroup
+            const size_t m = (size_t)std::atoi(e) * 1024;
+            if (m > st.lds && m <= 160 * 1024) st.lds = m;
+        }
         st.note = note;
         p.stages.push_back(st);
     }
@@ -675,6 +704,12 @@ void plan_build(Plan &p)
 
 // ====================================================================== device side
 
+static const void *stage_kernel(const Stage &s)
+{
+    if (s.sa.split_lds) return s.sign < 0 ? s.inst->rows_fwd : s.inst->rows_inv;
+    return s.sign < 0 ? s.inst->fwd : s.inst->inv;
+}
+
 void plan_materialize(Plan &p)
 {
     SIP_HIP_CHECK(hipGetDevice(&p.device));
@@ -690,7 +725,7 @@ void plan_materialize(Plan &p)
             s.sa.rtw = s.rtw.dev ? static_cast<const float2 *>(s.rtw.dev->ptr) : nullptr;
             s.sa.tw4_lo = s.tw4lo.dev ? static_cast<const float2 *>(s.tw4lo.dev->ptr) : nullptr;
             s.sa.tw4_hi = s.tw4hi.dev ? static_cast<const float2 *>(s.tw4hi.dev->ptr) : nullptr;
-            const void *fn = s.sign < 0 ? s.inst->fwd : s.inst->inv;
+            const void *fn = stage_kernel(s);
             if (s.lds > 64 * 1024)
                 SIP_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s.lds));
         } else {
@@ -721,7 +756,7 @@ void plan_execute(Plan &p, const void *in, void *out)
             if (a.tin.vec2 && (reinterpret_cast<uintptr_t>(src) & 15)) a.tin.vec2 = 0;
             if (a.tout.vec2 && (reinterpret_cast<uintptr_t>(dst) & 15)) a.tout.vec2 = 0;
             void *args[] = {&a};
-            const void *fn = s.sign < 0 ? s.inst->fwd : s.inst->inv;
+            const void *fn = stage_kernel(s);
             SIP_HIP_CHECK(hipLaunchKernel(fn, s.grid, s.block, args, s.lds, p.stream));
         } else {
             AuxArgs a = s.xa;
@@ -756,7 +791,10 @@ std::string plan_describe(const Plan &p)
             for (int r = 0; r < s.inst->npass; r++) os << (r ? "," : "") << s.inst->radix[r];
             os << "],\"nfft\":" << s.sa.nfft << ",\"io\":" << s.sa.io_mode << ",\"lds\":" << s.lds
                << ",\"grid\":" << s.grid.x << ",\"block\":" << s.block.x << ",\"unit_f_in\":" << s.sa.tin.unit_f
-               << ",\"unit_f_out\":" << s.sa.tout.unit_f << ",\"tw4\":" << s.sa.tw4_ntot;
+               << ",\"unit_f_out\":" << s.sa.tout.unit_f << ",\"tw4\":" << s.sa.tw4_ntot
+               << ",\"direct_in\":" << s.sa.direct_in << ",\"direct_out\":" << s.sa.direct_out
+               << ",\"split_lds\":" << s.sa.split_lds
+               << ",\"xcd_remap\":" << s.sa.xcd_remap;
         } else {
             os << ",\"mode\":" << s.xa.mode << ",\"rows\":" << s.xa.rows << ",\"n\":" << s.xa.n
                << ",\"out_len\":" << s.xa.out_len;

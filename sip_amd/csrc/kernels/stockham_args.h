@@ -56,6 +56,10 @@ struct StockhamArgs {
     int32_t io_mode;
     int32_t in_len;        // elements per transform on input (L, L+1 bins, L reals ...)
     int32_t out_len;       // elements per transform on output
+    int32_t direct_in;     // first radix pass reads HBM directly (contiguous complex rows)
+    int32_t direct_out;    // last radix pass writes HBM directly (contiguous complex rows)
+    int32_t split_lds;     // launched as the single-plane row kernel (direct rows only)
+    int32_t xcd_remap;     // give each XCD a contiguous range of tiles
     FastDiv div_in_len;    // divides by in_len
     FastDiv div_out_len;   // divides by out_len
     FastDiv div_F;         // divides by F
@@ -74,6 +78,8 @@ struct StockhamInstance {
     int32_t radix[12];
     const void *fwd;         // __global__ function handles (hipLaunchKernel symbol)
     const void *inv;
+    const void *rows_fwd;    // single-plane contiguous-row kernels (T >= 16 only, else null)
+    const void *rows_inv;
 };
 
 }  // namespace sip

@@ -30,11 +30,72 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &d)
 
 __device__ __forceinline__ int lds_pad(int j) { return j + (j >> 4); }
 
+// Streaming HBM accesses of the direct passes (read once / written once).
+#ifndef SIP_NT
+#define SIP_NT 0
+#endif
+typedef float sip_vf2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 gload(const float2 *p)
+{
+#if SIP_NT
+    const sip_vf2 v = __builtin_nontemporal_load(reinterpret_cast<const sip_vf2 *>(p));
+    return make_float2(v.x, v.y);
+#else
+    return *p;
+#endif
+}
+__device__ __forceinline__ void gstore(float2 *p, float2 v)
+{
+#if SIP_NT
+    sip_vf2 w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<sip_vf2 *>(p));
+#else
+    *p = v;
+#endif
+}
+
+template <int A, int... Rest>
+constexpr int first_of()
+{
+    return A;
+}
+
+// Padded LDS index of element base + r * STRIDE for a compile-time STRIDE. When the padding of
+// the r-dependent part is a compile-time constant the result is pad(base) + const(r), so a
+// butterfly's R accesses share one address and differ only in the instruction's offset field.
+template <int STRIDE>
+__device__ __forceinline__ int lds_idx_read(int base, int r)
+{
+    if constexpr (STRIDE % 16 == 0)
+        return lds_pad(base) + r * (STRIDE + STRIDE / 16);
+    else
+        return lds_pad(base + r * STRIDE);
+}
+
+// Padded index of the Stockham output element hi*NS*R + k + r*NS (k < NS).
+template <int NS, int R>
+__device__ __forceinline__ int lds_idx_write(int hi, int k, int r)
+{
+    if constexpr (NS % 16 == 0) {
+        return lds_pad(hi * NS * R + k) + r * (NS + NS / 16);
+    } else if constexpr (16 % NS == 0 && (NS * R) % 16 == 0) {
+        // k + r*NS never crosses a multiple of 16 inside one r step because NS divides 16
+        return hi * (NS * R + NS * R / 16) + k + r * NS + (r * NS) / 16;
+    } else {
+        return lds_pad(hi * NS * R + k + r * NS);
+    }
+}
+
 // One radix-R pass over a transform of length L held in LDS, T threads per transform,
 // NS = product of the radices of the earlier passes, TW = this pass's twiddles [k][r-1].
-template <int L, int T, int S, int NS, int R>
+// GIN: the pass reads its inputs straight from HBM (first pass of a contiguous row: element
+// b + r*L/R, coalesced across the transform's threads) instead of LDS. GOUT: the pass writes its
+// outputs straight to HBM (last pass: the autosort output index is b + r*L/R as well).
+template <int L, int T, int S, int NS, int R, bool GIN, bool GOUT>
 __device__ __forceinline__ void pass_static(float *re, float *im, const float2 *__restrict__ tw, int lbase, int t,
-                                            bool active)
+                                            bool active, const float2 *__restrict__ gin, float2 *__restrict__ gout)
 {
     constexpr int NBF = L / R;                 // butterflies per transform
     constexpr int NB = (NBF + T - 1) / T;      // butterflies per thread
@@ -47,8 +108,12 @@ __device__ __forceinline__ void pass_static(float *re, float *im, const float2 *
         if (active && (EXACT || b < NBF)) {
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                const int j = lds_pad(b + r * NBF);
-                v[q][r] = make_float2(re[lbase + j], im[lbase + j]);
+                if constexpr (GIN) {
+                    v[q][r] = gload(gin + b + r * NBF);
+                } else {
+                    const int j = lds_idx_read<NBF>(b, r);
+                    v[q][r] = make_float2(re[lbase + j], im[lbase + j]);
+                }
             }
             if constexpr (NS > 1) {
                 const int k = b % NS;
@@ -58,7 +123,7 @@ __device__ __forceinline__ void pass_static(float *re, float *im, const float2 *
             }
         }
     }
-    __syncthreads();
+    if constexpr (!GIN && !GOUT) __syncthreads();
 #pragma unroll
     for (int q = 0; q < NB; q++) {
         const int b = t + q * T;
@@ -69,25 +134,32 @@ __device__ __forceinline__ void pass_static(float *re, float *im, const float2 *
             }
             dft<R, S>(v[q]);
             const int k = b % NS;
-            const int d = (b / NS) * NS * R + k;
+            const int hi = b / NS;
+            const int d = hi * NS * R + k;
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                const int j = lds_pad(d + r * NS);
-                re[lbase + j] = v[q][r].x;
-                im[lbase + j] = v[q][r].y;
+                if constexpr (GOUT) {
+                    gstore(gout + d + r * NS, v[q][r]);
+                } else {
+                    const int j = lds_idx_write<NS, R>(hi, k, r);
+                    re[lbase + j] = v[q][r].x;
+                    im[lbase + j] = v[q][r].y;
+                }
             }
         }
     }
-    __syncthreads();
+    if constexpr (!GOUT) __syncthreads();
 }
 
-template <int L, int T, int S, int NS, int R, int... Rest>
+template <int L, int T, int S, bool DIN, bool DOUT, int NS, int R, int... Rest>
 __device__ __forceinline__ void passes_static(float *re, float *im, const float2 *__restrict__ tw, int lbase, int t,
-                                              bool active)
+                                              bool active, const float2 *__restrict__ gin, float2 *__restrict__ gout)
 {
-    pass_static<L, T, S, NS, R>(re, im, tw, lbase, t, active);
-    if constexpr (sizeof...(Rest) > 0)
-        passes_static<L, T, S, NS * R, Rest...>(re, im, tw + NS * (R - 1), lbase, t, active);
+    constexpr bool first = NS == 1;
+    constexpr bool last = sizeof...(Rest) == 0;
+    pass_static<L, T, S, NS, R, DIN && first, DOUT && last>(re, im, tw, lbase, t, active, gin, gout);
+    if constexpr (!last)
+        passes_static<L, T, S, DIN, DOUT, NS * R, Rest...>(re, im, tw + NS * (R - 1), lbase, t, active, gin, gout);
 }
 
 // f = f0 + n0 * (f1 + n1 * f2)
@@ -105,6 +177,135 @@ __device__ __forceinline__ float2 tw4_root(const StockhamArgs &a, int64_t m)
     return cmul(a.tw4_lo[m & a.tw4_mask], a.tw4_hi[m >> a.tw4_shift]);
 }
 
+// Contiguous C2C rows with one LDS plane per transform (half the LDS of the two-plane layout, so
+// twice as many workgroups fit on a CU). The first pass reads HBM, the last writes HBM, and every
+// exchange in between moves the real parts through the plane, then the imaginary parts.
+template <int L, int T, int S, int NS, int R, int... Rest>
+__device__ __forceinline__ void split_chain(float *pl, const float2 *__restrict__ tw, int lbase, int t, bool active,
+                                            float2 (&v)[(L / R + T - 1) / T][R], float2 *__restrict__ gout)
+{
+    constexpr int NBF = L / R;
+    constexpr int NB = (NBF + T - 1) / T;
+    constexpr bool EXACT = (NBF % T) == 0;
+    // this pass: twiddle + DFT in registers
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+        const int b = t + q * T;
+        if (active && (EXACT || b < NBF)) {
+            if constexpr (NS > 1) {
+                const int k = b % NS;
+                const float2 *tp = tw + k * (R - 1);
+#pragma unroll
+                for (int r = 1; r < R; r++) v[q][r] = cmul(v[q][r], tp[r - 1]);
+            }
+            dft<R, S>(v[q]);
+        }
+    }
+    if constexpr (sizeof...(Rest) == 0) {
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int b = t + q * T;
+            if (active && (EXACT || b < NBF)) {
+                const int k = b % NS;
+                const int d = (b / NS) * NS * R + k;
+#pragma unroll
+                for (int r = 0; r < R; r++) gstore(gout + d + r * NS, v[q][r]);
+            }
+        }
+    } else {
+        constexpr int R2 = first_of<Rest...>();
+        constexpr int NBF2 = L / R2;
+        constexpr int NB2 = (NBF2 + T - 1) / T;
+        constexpr bool EXACT2 = (NBF2 % T) == 0;
+        float2 n[NB2][R2];
+#pragma unroll
+        for (int plane = 0; plane < 2; plane++) {
+            if (plane == 1 || NS > 1) __syncthreads();  // previous readers of the plane are done
+#pragma unroll
+            for (int q = 0; q < NB; q++) {
+                const int b = t + q * T;
+                if (active && (EXACT || b < NBF)) {
+                    const int k = b % NS;
+                    const int hi = b / NS;
+#pragma unroll
+                    for (int r = 0; r < R; r++)
+                        pl[lbase + lds_idx_write<NS, R>(hi, k, r)] = plane ? v[q][r].y : v[q][r].x;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < NB2; q++) {
+                const int b = t + q * T;
+                if (active && (EXACT2 || b < NBF2)) {
+#pragma unroll
+                    for (int r = 0; r < R2; r++) {
+                        const float x = pl[lbase + lds_idx_read<NBF2>(b, r)];
+                        if (plane)
+                            n[q][r].y = x;
+                        else
+                            n[q][r].x = x;
+                    }
+                }
+            }
+        }
+        split_chain<L, T, S, NS * R, Rest...>(pl, tw + NS * (R - 1), lbase, t, active, n, gout);
+    }
+}
+
+template <int L, int T, int S, int R0, int... Rest>
+__device__ __forceinline__ void split_rows(const StockhamArgs &a, float *pl, int64_t tile)
+{
+    constexpr int NBF0 = L / R0;
+    constexpr int NB0 = (NBF0 + T - 1) / T;
+    constexpr bool EXACT0 = (NBF0 % T) == 0;
+    const int F = a.F;
+    const int tid = threadIdx.x;
+    const int flp = tid / T;
+    const int t = tid - flp * T;
+    const int64_t f = tile * F + flp;
+    const bool active = f < a.nfft;
+    int64_t ib = 0, ob = 0;
+    if (active) {
+        uint32_t f0, f1, f2;
+        tile_split((uint32_t)f, a.tin, f0, f1, f2);
+        ib = (int64_t)f0 * a.tin.s0 + (int64_t)f1 * a.tin.s1 + (int64_t)f2 * a.tin.s2;
+        tile_split((uint32_t)f, a.tout, f0, f1, f2);
+        ob = (int64_t)f0 * a.tout.s0 + (int64_t)f1 * a.tout.s1 + (int64_t)f2 * a.tout.s2;
+    }
+    const float2 *g = static_cast<const float2 *>(a.in) + ib;
+    float2 v[NB0][R0];
+#pragma unroll
+    for (int q = 0; q < NB0; q++) {
+        const int b = t + q * T;
+        if (active && (EXACT0 || b < NBF0)) {
+#pragma unroll
+            for (int r = 0; r < R0; r++) v[q][r] = gload(g + b + r * NBF0);
+        }
+    }
+    split_chain<L, T, S, 1, R0, Rest...>(pl, a.tw, flp * a.LP, t, active, v, static_cast<float2 *>(a.out) + ob);
+}
+
+// Tile of this workgroup; with xcd_remap the round-robin dispatch of consecutive workgroups to the
+// 8 XCDs is undone so that each XCD streams through one contiguous range of tiles.
+__device__ __forceinline__ int64_t tile_index(const StockhamArgs &a)
+{
+    int64_t tile = blockIdx.x;
+    if (a.xcd_remap) {
+        const int64_t ntiles = gridDim.x, q = ntiles / 8, rem = ntiles % 8;
+        const int64_t x = tile % 8, sl = tile / 8;
+        tile = x * q + (x < rem ? x : rem) + sl;
+    }
+    return tile;
+}
+
+// Contiguous C2C rows, single LDS plane (separate kernel so that its register budget is its own).
+template <int L, int T, int S, int... Rs>
+__global__ __launch_bounds__(1024) void stockham_rows_kernel(const StockhamArgs a)
+{
+    extern __shared__ float lds[];
+    split_rows<L, T, S, Rs...>(a, lds, tile_index(a));
+}
+
 template <int L, int T, int S, int... Rs>
 __global__ __launch_bounds__(1024) void stockham_kernel(const StockhamArgs a)
 {
@@ -119,7 +320,9 @@ __global__ __launch_bounds__(1024) void stockham_kernel(const StockhamArgs a)
     int64_t *out_base = in_base + F;
     int64_t *col = out_base + F;
 
-    const int64_t fstart = (int64_t)blockIdx.x * F;
+    const int64_t tile = tile_index(a);
+
+    const int64_t fstart = tile * F;
     const int nf = (int)((a.nfft - fstart) < F ? (a.nfft - fstart) : F);
 
     // Per-transform base offsets (and the four-step column index).
@@ -136,7 +339,9 @@ __global__ __launch_bounds__(1024) void stockham_kernel(const StockhamArgs a)
 
     // ---------------------------------------------------------------- load tile
     const int in_len = a.in_len;
-    if (a.io_mode == IO_R2C_FULL) {
+    if (a.direct_in) {
+        // the first radix pass reads HBM directly
+    } else if (a.io_mode == IO_R2C_FULL) {
         const float *src = static_cast<const float *>(a.in);
         const int total = F * in_len;
         for (int i = tid; i < total; i += nthreads) {
@@ -249,8 +454,19 @@ __global__ __launch_bounds__(1024) void stockham_kernel(const StockhamArgs a)
         const int t = tid - flp * T;
         const bool active = flp < nf;
         const int lbase = (active ? flp : 0) * LP;
-        passes_static<L, T, S, 1, Rs...>(re, im, a.tw, lbase, t, active);
+        const int fsel = active ? flp : 0;
+        const float2 *gin = static_cast<const float2 *>(a.in) + (a.direct_in ? in_base[fsel] : 0);
+        float2 *gout = static_cast<float2 *>(a.out) + (a.direct_out ? out_base[fsel] : 0);
+        if (a.direct_in && a.direct_out)
+            passes_static<L, T, S, true, true, 1, Rs...>(re, im, a.tw, lbase, t, active, gin, gout);
+        else if (a.direct_in)
+            passes_static<L, T, S, true, false, 1, Rs...>(re, im, a.tw, lbase, t, active, gin, gout);
+        else if (a.direct_out)
+            passes_static<L, T, S, false, true, 1, Rs...>(re, im, a.tw, lbase, t, active, gin, gout);
+        else
+            passes_static<L, T, S, false, false, 1, Rs...>(re, im, a.tw, lbase, t, active, gin, gout);
     }
+    if (a.direct_out) return;
 
     // ---------------------------------------------------------------- store tile
     const int out_len = a.out_len;
