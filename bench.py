@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--no-batching", action="store_true", help="skip the uneven-batch (batching form) leg")
     return ap.parse_args()
 
 
@@ -70,6 +71,90 @@ def make_handle(S, torch, dims, fft_type, batch, stream):
     h.alloc_workspace()
     assert h.set_stream(stream.cuda_stream) == 0
     return h
+
+
+def bench_batching(torch, S, stream, reps=5):
+    """Batching form: 4096 C2C problems, length log-uniform over 2^a 3^b 5^c in [64, 65536], batch
+    U[1, 64] (sip_amd.batching_workload). Compares one hipGraph launch of the length-grouped,
+    autotuned plan with (a) one plain asdFftExecC2C per problem and (b) the grouped plan launched
+    directly without tuning."""
+    import tempfile
+    lengths, batches = S.batching_workload(4096)
+    elems = [n * b for n, b in zip(lengths, batches)]
+    total = sum(elems)
+    flops = sum(5.0 * n * math.log2(n) * b for n, b in zip(lengths, batches))
+    nbytes = 16.0 * total
+    offs = [0]
+    for e in elems:
+        offs.append(offs[-1] + e)
+    with torch.cuda.stream(stream):
+        xin = torch.view_as_complex(torch.rand((total, 2), device="cuda") * 2 - 1).contiguous()
+        yout = torch.empty_like(xin)
+    torch.cuda.synchronize()
+    base_in, base_out = xin.data_ptr(), yout.data_ptr()
+    ins = [base_in + 8 * o for o in offs[:-1]]
+    outs = [base_out + 8 * o for o in offs[:-1]]
+
+    def run(fn):
+        with torch.cuda.stream(stream):
+            fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    res = {"problems": len(lengths), "distinct_lengths": len(set(lengths)), "transforms": sum(batches),
+           "algorithmic_bytes": int(nbytes), "flops_5NlogN": flops}
+    # (a) one plain plan + exec per problem
+    hs = []
+    for n, b in zip(lengths, batches):
+        h = S.FftHandle()
+        assert h.make_plan_1d(n, S.ASCEND_FFT_C2C, S.ASCEND_FFT_FORWARD, b) == 0
+        h.alloc_workspace()
+        assert h.set_stream(stream.cuda_stream) == 0
+        hs.append(h)
+    tens_in = [xin[offs[i]:offs[i + 1]].view(batches[i], lengths[i]) for i in range(len(lengths))]
+    tens_out = [yout[offs[i]:offs[i + 1]].view(batches[i], lengths[i]) for i in range(len(lengths))]
+
+    def per_problem():
+        for h, a, b in zip(hs, tens_in, tens_out):
+            assert h.exec_c2c(a, b) == 0
+    ms_a = run(per_problem)
+    for h in hs:
+        h.destroy()
+    # (b) grouped, direct launches, default tiles
+    bh = S.BatchHandle()
+    assert bh.make_plan(lengths, batches, ins, outs, S.ASCEND_FFT_FORWARD, 0) == 0
+    bh.alloc_workspace()
+    assert bh.set_stream(stream.cuda_stream) == 0
+    ms_b = run(lambda: bh.exec())
+    bh.destroy()
+    # (c) grouped + autotuned + one hipGraph launch
+    cache = os.path.join(tempfile.gettempdir(), "sip_fft_bench_tune.jsonl")
+    os.environ["SIP_FFT_TUNE_CACHE"] = cache
+    t0 = time.perf_counter()
+    bg = S.BatchHandle()
+    assert bg.make_plan(lengths, batches, ins, outs, S.ASCEND_FFT_FORWARD, S.BATCH_GRAPH | S.BATCH_AUTOTUNE) == 0
+    tune_s = time.perf_counter() - t0
+    bg.alloc_workspace()
+    assert bg.set_stream(stream.cuda_stream) == 0
+    ms_c = run(lambda: bg.exec())
+    d = bg.describe()
+    bg.destroy()
+    for name, ms in (("per_problem_plain", ms_a), ("grouped_direct", ms_b), ("graph_autotuned", ms_c)):
+        res[name] = {"ms": round(ms, 4), "GFLOP/s": round(flops / (ms * 1e-3) / 1e9, 1),
+                     "hbm_GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+    res["graph_vs_per_problem_speedup"] = round(ms_a / ms_c, 3)
+    res["groups"] = d["groups"]
+    res["chains"] = d["chains"]
+    res["plan_and_tune_s"] = round(tune_s, 2)
+    del xin, yout
+    torch.cuda.empty_cache()
+    return res
 
 
 def main():
@@ -174,6 +259,9 @@ def main():
             hh.destroy()
             del xi, yo
         out["secondary"] = sec
+
+    if not args.no_batching:
+        out["batching"] = bench_batching(torch, S, stream)
 
     if not args.no_cpu:
         import numpy as np

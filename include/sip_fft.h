@@ -76,6 +76,23 @@ int32_t sipFftPlanDescribeDry(int32_t ndim, const int64_t *dims, int32_t fftType
 /* Library version string. */
 const char *sipFftVersion(void);
 
+/* Batching form: `count` C2C problems of uneven lengths run as one launch. Problem i is
+ * batches[i] transforms of length lengths[i], contiguous complex64 at device address in_ptrs[i]
+ * (8-byte aligned), written to out_ptrs[i]. The buffers are bound at plan time.
+ * flags: bit 0 = capture the whole batch into one hipGraph (each exec is one graph launch);
+ *        bit 1 = autotune tile shapes per length group (decisions cached in $SIP_FFT_TUNE_CACHE,
+ *                default $HOME/.sip_fft_tune.jsonl).
+ * Exec only enqueues on the bound stream. Workspace: query after MakePlan, caller-allocated. */
+int32_t sipFftBatchCreate(void **handle);
+int32_t sipFftBatchSetStream(void *handle, void *stream);
+int32_t sipFftBatchMakePlan(void *handle, int32_t count, const int64_t *lengths, const int64_t *batches,
+                            void *const *in_ptrs, void *const *out_ptrs, int32_t direction, int32_t flags);
+int32_t sipFftBatchGetWorkspaceSize(void *handle, size_t *bytes);
+int32_t sipFftBatchSetWorkspace(void *handle, void *workspace);
+int32_t sipFftBatchExec(void *handle);
+int32_t sipFftBatchDescribe(void *handle, char *buf, size_t cap, size_t *needed);
+int32_t sipFftBatchDestroy(void *handle);
+
 #ifdef __cplusplus
 }
 #endif

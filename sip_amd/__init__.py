@@ -50,7 +50,12 @@ EXPORTED_SYMBOLS = [
     "asdFftExecC2CSeparated", "asdFftIstftMakePlan", "asdFftExecIstft", "asdFftSynchronize", "asdFftGetType",
     "asdFftDestroy", "aclCreateTensor", "aclDestroyTensor", "aclGetDataType", "aclGetViewShape",
     "aclFreeViewShape", "aclGetTensorData", "sipFftPlanDescribe", "sipFftPlanDescribeDry", "sipFftVersion",
+    "sipFftBatchCreate", "sipFftBatchSetStream", "sipFftBatchMakePlan", "sipFftBatchGetWorkspaceSize",
+    "sipFftBatchSetWorkspace", "sipFftBatchExec", "sipFftBatchDescribe", "sipFftBatchDestroy",
 ]
+
+BATCH_GRAPH = 1
+BATCH_AUTOTUNE = 2
 
 _lib = None
 
@@ -91,6 +96,14 @@ def lib():
         "asdFftDestroy": [vp],
         "sipFftPlanDescribe": [vp, ctypes.c_char_p, sz, P(sz)],
         "sipFftPlanDescribeDry": [i32, P(i64), i32, i32, i64, i32, ctypes.c_char_p, sz, P(sz)],
+        "sipFftBatchCreate": [P(vp)],
+        "sipFftBatchSetStream": [vp, vp],
+        "sipFftBatchMakePlan": [vp, i32, P(i64), P(i64), P(vp), P(vp), i32, i32],
+        "sipFftBatchGetWorkspaceSize": [vp, P(sz)],
+        "sipFftBatchSetWorkspace": [vp, vp],
+        "sipFftBatchExec": [vp],
+        "sipFftBatchDescribe": [vp, ctypes.c_char_p, sz, P(sz)],
+        "sipFftBatchDestroy": [vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -265,3 +278,92 @@ def describe_dry(dims, fft_type, direction, batch, dim_type=ASCEND_FFT_HORIZONTA
     arr = (ctypes.c_int64 * len(dims))(*dims)
     return json.loads(_describe(lambda buf, cap, need: lib().sipFftPlanDescribeDry(
         len(dims), arr, fft_type, direction, batch, dim_type, buf, cap, need)))
+
+
+class BatchHandle:
+    """Batching form (sipFftBatch*): uneven C2C problems bound to device buffers, one launch per exec."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(lib().sipFftBatchCreate(ctypes.byref(h)), "sipFftBatchCreate")
+        self.h = h
+        self._ws = None
+
+    def make_plan(self, lengths, batches, in_ptrs, out_ptrs, direction=ASCEND_FFT_FORWARD, flags=BATCH_GRAPH):
+        n = len(lengths)
+        L = (ctypes.c_int64 * n)(*[int(v) for v in lengths])
+        B = (ctypes.c_int64 * n)(*[int(v) for v in batches])
+        I = (ctypes.c_void_p * n)(*[int(v) for v in in_ptrs])
+        O = (ctypes.c_void_p * n)(*[int(v) for v in out_ptrs])
+        return lib().sipFftBatchMakePlan(self.h, n, L, B, I, O, direction, flags)
+
+    def workspace_size(self):
+        n = ctypes.c_size_t()
+        _check(lib().sipFftBatchGetWorkspaceSize(self.h, ctypes.byref(n)), "sipFftBatchGetWorkspaceSize")
+        return n.value
+
+    def alloc_workspace(self, device="cuda"):
+        import torch
+        n = self.workspace_size()
+        if n > 0:
+            self._ws = torch.empty(n, dtype=torch.uint8, device=device)
+            _check(lib().sipFftBatchSetWorkspace(self.h, ctypes.c_void_p(self._ws.data_ptr())),
+                   "sipFftBatchSetWorkspace")
+        return n
+
+    def set_stream(self, stream_ptr):
+        return lib().sipFftBatchSetStream(self.h, ctypes.c_void_p(int(stream_ptr)))
+
+    def exec(self):
+        return lib().sipFftBatchExec(self.h)
+
+    def describe(self) -> dict:
+        return json.loads(_describe(lambda buf, cap, need: lib().sipFftBatchDescribe(self.h, buf, cap, need)))
+
+    def destroy(self):
+        if self.h is not None:
+            st = lib().sipFftBatchDestroy(self.h)
+            self.h = None
+            self._ws = None
+            return st
+        return ACL_SUCCESS
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def smooth_lengths(lo, hi):
+    """All 2^a 3^b 5^c in [lo, hi], ascending."""
+    out = []
+    a = 1
+    while a <= hi:
+        b = a
+        while b <= hi:
+            c = b
+            while c <= hi:
+                if c >= lo:
+                    out.append(c)
+                c *= 5
+            b *= 3
+        a *= 2
+    return sorted(out)
+
+
+def batching_workload(count=4096, seed=0x5EED0004, lo=64, hi=65536, max_batch=64):
+    """The batching-form problem set: `count` C2C problems, length log-uniform over the 2,3,5-smooth
+    numbers in [lo, hi] (x = exp(U[ln lo, ln hi]), rounded up to the next smooth length), batch
+    uniform in [1, max_batch]; seeded (numpy PCG64)."""
+    import math
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sm = smooth_lengths(lo, hi)
+    lengths, batches = [], []
+    for _ in range(count):
+        x = math.exp(rng.uniform(math.log(lo), math.log(hi)))
+        n = next(v for v in sm if v >= x) if x <= hi else hi
+        lengths.append(int(n))
+        batches.append(int(rng.integers(1, max_batch + 1)))
+    return lengths, batches

@@ -1,0 +1,548 @@
+// Batching form: many C2C FFT problems of uneven lengths executed as one hipGraph launch.
+//
+// Problem i is batches[i] transforms of length lengths[i], stored contiguously at in_ptrs[i]
+// (written to out_ptrs[i]). Problems of the same length are grouped into one gathered-row plan
+// (plan_build_gathered_c2c): a single Stockham launch (or four-step pair) walks the rows of every
+// member through a device row-offset table, so no data is copied or packed. The groups are spread
+// over a few parallel chains (longest-processing-time first) and captured once into a hipGraph;
+// each exec is one graph launch. An optional autotuner times the runtime tile-shape choices of every
+// group (transforms per workgroup, LDS layout, tile order, four-step split) on the real buffers and
+// keeps the fastest, caching decisions in a JSON file keyed by length, row count, direction and
+// device. This layer has no counterpart in the reference; it is an extension API (sipFftBatch*).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <unordered_map>
+
+#include "plan.h"
+
+namespace sip {
+namespace {
+
+struct Group {
+    int64_t n = 0;
+    std::vector<int> members;
+    int64_t rows = 0;
+    double cost = 0;  // estimated bytes moved
+    std::unique_ptr<Plan> plan;
+    int chain = 0;
+    float tuned_us = -1.f;
+};
+
+struct BatchPlan {
+    int direction = kDirForward;
+    std::vector<int64_t> lengths, batches;
+    std::vector<void *> ins, outs;
+    std::vector<Group> groups;
+    int nchains = 1;
+    std::vector<size_t> chain_ws;      // workspace bytes per chain
+    std::vector<size_t> chain_ws_off;  // offset of each chain's region
+    size_t ws_bytes = 0;
+    void *workspace = nullptr;
+    hipStream_t stream = nullptr;
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipStream_t> side;  // capture-only side streams
+    std::vector<hipEvent_t> events;
+    hipStream_t own = nullptr;      // capture stream when the caller's stream is the null stream
+    hipEvent_t own_in = nullptr, own_out = nullptr;
+    std::mutex mu;
+    bool ready = false;
+
+    ~BatchPlan()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        for (hipStream_t s : side) (void)hipStreamDestroy(s);
+        if (own_in) (void)hipEventDestroy(own_in);
+        if (own_out) (void)hipEventDestroy(own_out);
+        if (own) (void)hipStreamDestroy(own);
+    }
+
+    // The null stream cannot be captured: graphs are captured on (and launched into) a private
+    // stream ordered after / before the caller's null-stream work with events.
+    hipStream_t work_stream() const { return stream ? stream : own; }
+};
+
+std::mutex g_batch_mu;
+std::unordered_map<void *, std::unique_ptr<BatchPlan>> g_batches;
+
+BatchPlan *lookup_batch(void *h)
+{
+    std::lock_guard<std::mutex> lk(g_batch_mu);
+    auto it = g_batches.find(h);
+    return it == g_batches.end() ? nullptr : it->second.get();
+}
+
+// ------------------------------------------------------------------ tuning cache (JSON lines)
+
+std::string cache_path()
+{
+    if (const char *e = std::getenv("SIP_FFT_TUNE_CACHE")) return e;
+    const char *home = std::getenv("HOME");
+    return std::string(home ? home : "/tmp") + "/.sip_fft_tune.jsonl";
+}
+
+std::string device_name()
+{
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return "unknown";
+    return prop.gcnArchName;
+}
+
+std::string tune_key(int64_t n, int64_t rows, int direction)
+{
+    int64_t bucket = 1;
+    while (bucket < rows) bucket <<= 1;
+    std::ostringstream os;
+    os << "c2c:" << n << ":" << bucket << ":" << direction << ":" << device_name();
+    return os.str();
+}
+
+std::mutex g_cache_mu;
+std::map<std::string, Tune> g_tune_cache;
+bool g_cache_loaded = false;
+
+void load_cache()
+{
+    if (g_cache_loaded) return;
+    g_cache_loaded = true;
+    std::ifstream f(cache_path());
+    std::string line;
+    while (std::getline(f, line)) {
+        char key[256];
+        Tune t;
+        long long l1 = 0;
+        if (std::sscanf(line.c_str(),
+                        "{\"key\":\"%255[^\"]\",\"row_F\":%d,\"split_lds\":%d,\"xcd\":%d,\"split_l1\":%lld,\"col_F\":%d}",
+                        key, &t.row_F, &t.split_lds, &t.xcd, &l1, &t.col_F) == 6) {
+            t.split_l1 = l1;
+            g_tune_cache[key] = t;
+        }
+    }
+}
+
+void store_cache(const std::string &key, const Tune &t)
+{
+    g_tune_cache[key] = t;
+    std::ofstream f(cache_path(), std::ios::app);
+    if (!f) return;  // an unwritable cache only costs re-tuning
+    f << "{\"key\":\"" << key << "\",\"row_F\":" << t.row_F << ",\"split_lds\":" << t.split_lds << ",\"xcd\":" << t.xcd
+      << ",\"split_l1\":" << t.split_l1 << ",\"col_F\":" << t.col_F << "}\n";
+}
+
+// ------------------------------------------------------------------ group plans
+
+std::unique_ptr<Plan> build_group_plan(const BatchPlan &bp, const Group &g, const Tune &t)
+{
+    auto p = std::make_unique<Plan>();
+    p->direction = bp.direction;
+    p->tune = t;
+    // element offsets relative to the lowest member pointer of each side
+    uintptr_t in_lo = UINTPTR_MAX, out_lo = UINTPTR_MAX;
+    for (int i : g.members) {
+        in_lo = std::min(in_lo, reinterpret_cast<uintptr_t>(bp.ins[(size_t)i]));
+        out_lo = std::min(out_lo, reinterpret_cast<uintptr_t>(bp.outs[(size_t)i]));
+    }
+    std::vector<int64_t> in_rows, out_rows;
+    in_rows.reserve((size_t)g.rows);
+    out_rows.reserve((size_t)g.rows);
+    for (int i : g.members) {
+        const int64_t ib = (int64_t)((reinterpret_cast<uintptr_t>(bp.ins[(size_t)i]) - in_lo) / 8);
+        const int64_t ob = (int64_t)((reinterpret_cast<uintptr_t>(bp.outs[(size_t)i]) - out_lo) / 8);
+        for (int64_t r = 0; r < bp.batches[(size_t)i]; r++) {
+            in_rows.push_back(ib + r * g.n);
+            out_rows.push_back(ob + r * g.n);
+        }
+    }
+    plan_build_gathered_c2c(*p, g.n, in_rows, out_rows, reinterpret_cast<void *>(in_lo),
+                            reinterpret_cast<void *>(out_lo));
+    plan_materialize(*p);
+    p->initialized = true;
+    return p;
+}
+
+// Candidate tile-shape choices for one group.
+std::vector<Tune> candidates(int64_t n)
+{
+    std::vector<Tune> c;
+    c.push_back(Tune{});
+    const StockhamInstance *inst = find_instance(n);
+    if (inst && n <= 16384) {
+        for (int sl : {0, 1})
+            for (int x : {0, 1}) {
+                if (sl == 1 && !inst->rows_fwd) continue;
+                Tune t;
+                t.split_lds = sl;
+                t.xcd = x;
+                c.push_back(t);
+            }
+        const int T = inst->T;
+        for (int f : {1, 2, 4, 8, 16}) {
+            if (f * T > 1024 || f * T < 64) continue;
+            Tune t;
+            t.row_F = f;
+            t.split_lds = 0;
+            c.push_back(t);
+        }
+    } else {
+        for (int64_t a = 2; a <= 1024 && a < n; a++) {
+            if (n % a || !find_instance(a)) continue;
+            const int64_t b = n / a;
+            if (!find_instance(b) && b > 16384) continue;
+            if (a * 16 < b || b * 16 < a) continue;  // keep splits within 16x of balanced
+            for (int cf : {0, 16, 64}) {
+                Tune t;
+                t.split_l1 = a;
+                t.col_F = cf;
+                c.push_back(t);
+            }
+            if (c.size() > 24) break;
+        }
+    }
+    return c;
+}
+
+float time_plan(Plan &p, hipStream_t s, int reps)
+{
+    hipEvent_t e0, e1;
+    SIP_HIP_CHECK(hipEventCreate(&e0));
+    SIP_HIP_CHECK(hipEventCreate(&e1));
+    p.stream = s;
+    plan_execute(p, nullptr, nullptr);  // warm-up
+    SIP_HIP_CHECK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; r++) plan_execute(p, nullptr, nullptr);
+    SIP_HIP_CHECK(hipEventRecord(e1, s));
+    SIP_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    SIP_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms * 1000.f / reps;
+}
+
+void tune_group(BatchPlan &bp, Group &g, void *scratch_ws, size_t scratch_bytes)
+{
+    const std::string key = tune_key(g.n, g.rows, bp.direction);
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        load_cache();
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            g.plan = build_group_plan(bp, g, it->second);
+            return;
+        }
+    }
+    Tune best;
+    float best_us = 1e30f;
+    for (const Tune &t : candidates(g.n)) {
+        std::unique_ptr<Plan> p;
+        try {
+            p = build_group_plan(bp, g, t);
+        } catch (const SipError &) {
+            continue;
+        }
+        if (p->ws_bytes > scratch_bytes) continue;
+        p->workspace = scratch_ws;
+        const float us = time_plan(*p, bp.stream, 5);  // null stream is fine for timing
+        if (us < best_us) {
+            best_us = us;
+            best = t;
+        }
+    }
+    g.tuned_us = best_us;
+    g.plan = build_group_plan(bp, g, best);
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    store_cache(key, best);
+}
+
+void capture(BatchPlan &bp)
+{
+    // Chains run concurrently: chain 0 on the plan's stream, the others on side streams forked and
+    // joined with events, so the graph holds nchains independent branches.
+    if (bp.exec) {
+        (void)hipGraphExecDestroy(bp.exec);
+        bp.exec = nullptr;
+    }
+    if (bp.graph) {
+        (void)hipGraphDestroy(bp.graph);
+        bp.graph = nullptr;
+    }
+    for (Group &g : bp.groups) g.plan->workspace = static_cast<char *>(bp.workspace) + bp.chain_ws_off[(size_t)g.chain];
+    auto enqueue_all = [&](bool graph_mode) {
+        hipStream_t main = bp.work_stream();
+        if (bp.nchains > 1) {
+            SIP_HIP_CHECK(hipEventRecord(bp.events[0], main));
+            for (int c = 1; c < bp.nchains; c++) SIP_HIP_CHECK(hipStreamWaitEvent(bp.side[(size_t)c - 1], bp.events[0], 0));
+        }
+        for (Group &g : bp.groups) {
+            g.plan->stream = g.chain == 0 ? main : bp.side[(size_t)g.chain - 1];
+            plan_execute(*g.plan, nullptr, nullptr);
+        }
+        if (bp.nchains > 1) {
+            for (int c = 1; c < bp.nchains; c++) {
+                SIP_HIP_CHECK(hipEventRecord(bp.events[(size_t)c], bp.side[(size_t)c - 1]));
+                SIP_HIP_CHECK(hipStreamWaitEvent(main, bp.events[(size_t)c], 0));
+            }
+        }
+        (void)graph_mode;
+    };
+    if (!bp.use_graph) {
+        enqueue_all(false);
+        return;
+    }
+    hipStream_t cs = bp.work_stream();
+    SIP_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueue_all(true);
+    } catch (...) {
+        hipGraph_t dead = nullptr;
+        (void)hipStreamEndCapture(cs, &dead);
+        if (dead) (void)hipGraphDestroy(dead);
+        throw;
+    }
+    SIP_HIP_CHECK(hipStreamEndCapture(cs, &bp.graph));
+    SIP_HIP_CHECK(hipGraphInstantiate(&bp.exec, bp.graph, nullptr, nullptr, 0));
+}
+
+template <class F>
+int32_t guarded(F &&f)
+{
+    try {
+        return f();
+    } catch (const SipError &e) {
+        SIP_LOG(ERROR, e.what());
+        return e.status;
+    } catch (const std::exception &e) {
+        SIP_LOG(ERROR, e.what());
+        return AsdSip::ACL_ERROR_INTERNAL_ERROR;
+    } catch (...) {
+        return AsdSip::ACL_ERROR_INTERNAL_ERROR;
+    }
+}
+
+}  // namespace
+}  // namespace sip
+
+using sip::BatchPlan;
+namespace E = AsdSip;
+
+extern "C" {
+
+int32_t sipFftBatchCreate(void **handle)
+{
+    return sip::guarded([&]() -> int32_t {
+        if (!handle) return E::ACL_ERROR_INVALID_PARAM;
+        auto b = std::make_unique<BatchPlan>();
+        void *h = b.get();
+        std::lock_guard<std::mutex> lk(sip::g_batch_mu);
+        sip::g_batches.emplace(h, std::move(b));
+        *handle = h;
+        return E::ACL_SUCCESS;
+    });
+}
+
+int32_t sipFftBatchSetStream(void *handle, void *stream)
+{
+    return sip::guarded([&]() -> int32_t {
+        BatchPlan *b = sip::lookup_batch(handle);
+        if (!b) return E::ACL_ERROR_INVALID_PARAM;
+        std::lock_guard<std::mutex> lk(b->mu);
+        b->stream = static_cast<hipStream_t>(stream);
+        b->ready = false;
+        return E::ACL_SUCCESS;
+    });
+}
+
+int32_t sipFftBatchMakePlan(void *handle, int32_t count, const int64_t *lengths, const int64_t *batches,
+                            void *const *in_ptrs, void *const *out_ptrs, int32_t direction, int32_t flags)
+{
+    return sip::guarded([&]() -> int32_t {
+        BatchPlan *b = sip::lookup_batch(handle);
+        if (!b || count <= 0 || !lengths || !batches || !in_ptrs || !out_ptrs) return E::ACL_ERROR_INVALID_PARAM;
+        if (direction != sip::kDirForward && direction != sip::kDirInverse) return E::ACL_ERROR_INVALID_PARAM;
+        std::lock_guard<std::mutex> lk(b->mu);
+        b->direction = direction;
+        b->lengths.assign(lengths, lengths + count);
+        b->batches.assign(batches, batches + count);
+        b->ins.assign(in_ptrs, in_ptrs + count);
+        b->outs.assign(out_ptrs, out_ptrs + count);
+        b->use_graph = (flags & 1) != 0;
+        for (int i = 0; i < count; i++) {
+            if (lengths[i] <= 0 || lengths[i] > sip::kMaxFftSize || batches[i] <= 0) return E::ACL_ERROR_INVALID_PARAM;
+            if (!in_ptrs[i] || !out_ptrs[i]) return E::ACL_ERROR_INVALID_PARAM;
+            if ((reinterpret_cast<uintptr_t>(in_ptrs[i]) | reinterpret_cast<uintptr_t>(out_ptrs[i])) & 7)
+                return E::ACL_ERROR_INVALID_PARAM;
+        }
+        // group by length
+        std::map<int64_t, sip::Group> by;
+        for (int i = 0; i < count; i++) {
+            sip::Group &g = by[lengths[i]];
+            g.n = lengths[i];
+            g.members.push_back(i);
+            g.rows += batches[i];
+        }
+        b->groups.clear();
+        for (auto &kv : by) {
+            kv.second.cost = 16.0 * (double)kv.second.n * (double)kv.second.rows;
+            b->groups.push_back(std::move(kv.second));
+        }
+        // chains: longest processing time first
+        b->nchains = (int)std::min<size_t>(4, b->groups.size());
+        std::sort(b->groups.begin(), b->groups.end(), [](const sip::Group &x, const sip::Group &y) { return x.cost > y.cost; });
+        std::vector<double> load((size_t)b->nchains, 0.0);
+        for (sip::Group &g : b->groups) {
+            const size_t c = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+            g.chain = (int)c;
+            load[c] += g.cost + 2.0e5;  // + a fixed per-launch cost
+        }
+        // plans (tuned or default); tuning needs a scratch workspace sized for the largest candidate
+        const bool autotune = (flags & 2) != 0;
+        void *scratch = nullptr;
+        size_t scratch_bytes = 0;
+        if (autotune) {
+            for (sip::Group &g : b->groups) scratch_bytes = std::max(scratch_bytes, (size_t)(16.0 * g.n * g.rows) + 4096);
+            SIP_HIP_CHECK(hipMalloc(&scratch, scratch_bytes));
+        }
+        try {
+            for (sip::Group &g : b->groups) {
+                if (autotune)
+                    sip::tune_group(*b, g, scratch, scratch_bytes);
+                else
+                    g.plan = sip::build_group_plan(*b, g, sip::Tune{});
+            }
+        } catch (...) {
+            if (scratch) (void)hipFree(scratch);
+            throw;
+        }
+        if (scratch) SIP_HIP_CHECK(hipFree(scratch));
+        // workspace: chains run concurrently, groups within a chain sequentially
+        b->chain_ws.assign((size_t)b->nchains, 0);
+        for (sip::Group &g : b->groups)
+            b->chain_ws[(size_t)g.chain] = std::max(b->chain_ws[(size_t)g.chain], g.plan->ws_bytes);
+        b->chain_ws_off.assign((size_t)b->nchains, 0);
+        size_t off = 0;
+        for (int c = 0; c < b->nchains; c++) {
+            b->chain_ws_off[(size_t)c] = off;
+            off += (b->chain_ws[(size_t)c] + 255) / 256 * 256;
+        }
+        b->ws_bytes = off;
+        b->ready = false;
+        return E::ACL_SUCCESS;
+    });
+}
+
+int32_t sipFftBatchGetWorkspaceSize(void *handle, size_t *bytes)
+{
+    BatchPlan *b = sip::lookup_batch(handle);
+    if (!b || !bytes) return E::ACL_ERROR_INVALID_PARAM;
+    std::lock_guard<std::mutex> lk(b->mu);
+    *bytes = b->ws_bytes;
+    return E::ACL_SUCCESS;
+}
+
+int32_t sipFftBatchSetWorkspace(void *handle, void *ws)
+{
+    BatchPlan *b = sip::lookup_batch(handle);
+    if (!b) return E::ACL_ERROR_INVALID_PARAM;
+    std::lock_guard<std::mutex> lk(b->mu);
+    b->workspace = ws;
+    b->ready = false;
+    return E::ACL_SUCCESS;
+}
+
+int32_t sipFftBatchExec(void *handle)
+{
+    return sip::guarded([&]() -> int32_t {
+        BatchPlan *b = sip::lookup_batch(handle);
+        if (!b) return E::ACL_ERROR_INVALID_PARAM;
+        std::lock_guard<std::mutex> lk(b->mu);
+        if (b->groups.empty()) return E::ACL_ERROR_INVALID_PARAM;
+        if (b->ws_bytes > 0 && !b->workspace) return E::ACL_ERROR_INTERNAL_ERROR;
+        if (b->side.size() + 1 < (size_t)b->nchains) {
+            for (int c = (int)b->side.size() + 1; c < b->nchains; c++) {
+                hipStream_t s;
+                SIP_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+                b->side.push_back(s);
+            }
+            while (b->events.size() < (size_t)b->nchains) {
+                hipEvent_t e;
+                SIP_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                b->events.push_back(e);
+            }
+        }
+        if (!b->use_graph) {
+            sip::capture(*b);  // direct launches
+            return E::ACL_SUCCESS;
+        }
+        if (!b->stream && !b->own) {
+            SIP_HIP_CHECK(hipStreamCreateWithFlags(&b->own, hipStreamNonBlocking));
+            SIP_HIP_CHECK(hipEventCreateWithFlags(&b->own_in, hipEventDisableTiming));
+            SIP_HIP_CHECK(hipEventCreateWithFlags(&b->own_out, hipEventDisableTiming));
+        }
+        if (!b->ready) {
+            sip::capture(*b);
+            b->ready = true;
+        }
+        if (b->stream) {
+            SIP_HIP_CHECK(hipGraphLaunch(b->exec, b->stream));
+        } else {
+            SIP_HIP_CHECK(hipEventRecord(b->own_in, nullptr));
+            SIP_HIP_CHECK(hipStreamWaitEvent(b->own, b->own_in, 0));
+            SIP_HIP_CHECK(hipGraphLaunch(b->exec, b->own));
+            SIP_HIP_CHECK(hipEventRecord(b->own_out, b->own));
+            SIP_HIP_CHECK(hipStreamWaitEvent(nullptr, b->own_out, 0));
+        }
+        return E::ACL_SUCCESS;
+    });
+}
+
+int32_t sipFftBatchDescribe(void *handle, char *buf, size_t cap, size_t *needed)
+{
+    return sip::guarded([&]() -> int32_t {
+        BatchPlan *b = sip::lookup_batch(handle);
+        if (!b) return E::ACL_ERROR_INVALID_PARAM;
+        std::lock_guard<std::mutex> lk(b->mu);
+        std::ostringstream os;
+        os << "{\"problems\":" << b->lengths.size() << ",\"groups\":" << b->groups.size() << ",\"chains\":" << b->nchains
+           << ",\"graph\":" << (b->use_graph ? 1 : 0) << ",\"workspace\":" << b->ws_bytes << ",\"group_list\":[";
+        for (size_t i = 0; i < b->groups.size(); i++) {
+            const sip::Group &g = b->groups[i];
+            os << (i ? "," : "") << "{\"n\":" << g.n << ",\"rows\":" << g.rows << ",\"chain\":" << g.chain
+               << ",\"stages\":" << g.plan->stages.size() << ",\"row_F\":" << g.plan->tune.row_F
+               << ",\"split_lds\":" << g.plan->tune.split_lds << ",\"xcd\":" << g.plan->tune.xcd
+               << ",\"split_l1\":" << g.plan->tune.split_l1 << ",\"col_F\":" << g.plan->tune.col_F
+               << ",\"tuned_us\":" << g.tuned_us << "}";
+        }
+        os << "]}";
+        const std::string s = os.str();
+        if (needed) *needed = s.size() + 1;
+        if (!buf || cap < s.size() + 1) return E::ACL_ERROR_INVALID_PARAM;
+        std::memcpy(buf, s.c_str(), s.size() + 1);
+        return E::ACL_SUCCESS;
+    });
+}
+
+int32_t sipFftBatchDestroy(void *handle)
+{
+    std::unique_ptr<BatchPlan> victim;
+    {
+        std::lock_guard<std::mutex> lk(sip::g_batch_mu);
+        auto it = sip::g_batches.find(handle);
+        if (it == sip::g_batches.end()) return E::ACL_ERROR_INVALID_PARAM;
+        victim = std::move(it->second);
+        sip::g_batches.erase(it);
+    }
+    return E::ACL_SUCCESS;
+}
+
+}  // extern "C"

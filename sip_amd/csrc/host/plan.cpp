@@ -196,6 +196,7 @@ namespace {
 
 struct Dim {
     int64_t n, s;
+    int tab = -1;  // row-table id: offset = table[index] instead of index * s
 };
 
 // A length-L axis of an array: element j of transform f sits at base(f) + j * es, with the
@@ -210,10 +211,10 @@ std::vector<Dim> normalize(const std::vector<Dim> &in)
 {
     std::vector<Dim> d;
     for (const Dim &x : in)
-        if (x.n != 1) d.push_back(x);
+        if (x.n != 1 || x.tab >= 0) d.push_back(x);
     std::vector<Dim> m;
     for (const Dim &x : d) {
-        if (!m.empty() && x.s == m.back().n * m.back().s) {
+        if (!m.empty() && x.tab < 0 && m.back().tab < 0 && x.s == m.back().n * m.back().s) {
             m.back().n *= x.n;
         } else {
             m.push_back(x);
@@ -221,6 +222,13 @@ std::vector<Dim> normalize(const std::vector<Dim> &in)
     }
     if (m.empty()) m.push_back({1, 0});
     return m;
+}
+
+bool has_table(const std::vector<Dim> &d)
+{
+    for (const Dim &x : d)
+        if (x.tab >= 0) return true;
+    return false;
 }
 
 int64_t count(const std::vector<Dim> &d)
@@ -233,8 +241,8 @@ int64_t count(const std::vector<Dim> &d)
 // Insert the dim for a split axis so that dims stay ordered by stride (unit stride first).
 size_t insert_pos(const std::vector<Dim> &dims, int64_t stride)
 {
-    size_t pos = 0;
-    while (pos < dims.size() && dims[pos].s < stride) pos++;
+    size_t pos = 0;  // gathered (table) dimensions count as the outermost
+    while (pos < dims.size() && dims[pos].tab < 0 && dims[pos].s < stride) pos++;
     return pos;
 }
 
@@ -244,10 +252,11 @@ std::vector<Dim> with_dim(std::vector<Dim> dims, size_t pos, Dim d)
     return dims;
 }
 
-TileDesc to_tile(const Lay &l, bool *ok)
+TileDesc to_tile(const Lay &l, bool *ok, int *tab_id)
 {
     std::vector<Dim> d = normalize(l.dims);
     TileDesc t{};
+    *tab_id = -1;
     if (d.size() > 3) {
         *ok = false;
         return t;
@@ -259,8 +268,18 @@ TileDesc to_tile(const Lay &l, bool *ok)
     t.s1 = d[1].s;
     t.s2 = d[2].s;
     t.es = l.es;
-    t.unit_f = (d[0].s == 1 && d[0].n > 1 && l.es != 1) ? 1 : 0;
-    t.vec2 = (l.es == 1 && (d[0].s % 2 == 0) && (d[1].s % 2 == 0) && (d[2].s % 2 == 0)) ? 1 : 0;
+    t.tab = nullptr;
+    t.tab_dim = -1;
+    for (int i = 0; i < 3; i++) {
+        if (d[(size_t)i].tab >= 0) {
+            if (*tab_id >= 0) *ok = false;  // one gathered dimension per tile
+            *tab_id = d[(size_t)i].tab;
+            t.tab_dim = i;
+        }
+    }
+    t.unit_f = (d[0].s == 1 && d[0].n > 1 && l.es != 1 && d[0].tab < 0) ? 1 : 0;
+    // gathered rows keep 8-byte accesses (row offsets need not be even)
+    t.vec2 = (*tab_id < 0 && l.es == 1 && (d[0].s % 2 == 0) && (d[1].s % 2 == 0) && (d[2].s % 2 == 0)) ? 1 : 0;
     t.div_n0 = make_fastdiv((uint32_t)t.n0);
     t.div_n1 = make_fastdiv((uint32_t)t.n1);
     return t;
@@ -337,8 +356,8 @@ struct Builder {
         st.sign = sign;
         StockhamArgs &a = st.sa;
         std::memset(&a, 0, sizeof(a));
-        a.tin = to_tile(in, &ok);
-        a.tout = to_tile(out, &ok);
+        a.tin = to_tile(in, &ok, &st.in_tab);
+        a.tout = to_tile(out, &ok, &st.out_tab);
         if (!ok) throw SipError(AsdSip::ACL_ERROR_API_NOT_SUPPORT, "layout has more than 3 batch dimensions");
         const int64_t nfft = count(normalize(in.dims));
         if (nfft != count(normalize(out.dims)))
@@ -358,16 +377,17 @@ struct Builder {
         const size_t per = (size_t)2 * lp * 4 + 24;
         int F;
         if (column) {
-            F = std::max(16, 256 / T);
-            F = std::min(F, 256);
+            // aim at >= 256-byte runs per row of the tile (32 transforms), within 80 KiB of LDS
+            F = p.tune.col_F > 0 ? p.tune.col_F : std::min(64, std::max(32, 1024 / T));
             while (F > 1 && ((size_t)F * per > 80 * 1024 || F * T > 1024)) F /= 2;
         } else {
             F = std::max(1, 256 / T);
             while (F > 1 && ((size_t)F * per > 48 * 1024 || F * T > 1024)) F--;
-            if (const char *e = std::getenv("SIP_FFT_ROW_F")) {
-                const int f = std::atoi(e);
-                if (f >= 1 && f * T <= 1024) F = f;
+            int f = p.tune.row_F;
+            if (f <= 0) {
+                if (const char *e = std::getenv("SIP_FFT_ROW_F")) f = std::atoi(e);
             }
+            if (f >= 1 && f * T <= 1024) F = f;
         }
         if (F > nfft) F = (int)nfft;
         if (F < 1) F = 1;
@@ -391,8 +411,8 @@ struct Builder {
         // single-plane exchange kernel (twice the resident workgroups) and XCD-contiguous tile order;
         // both stay available as tuning switches SIP_FFT_SPLIT_LDS=1 and SIP_FFT_XCD=1.
         const bool rows = io_mode == IO_C2C && a.direct_in && a.direct_out && inst->rows_fwd != nullptr;
-        a.split_lds = rows && env_flag("SIP_FFT_SPLIT_LDS", false);
-        a.xcd_remap = env_flag("SIP_FFT_XCD", false);
+        a.split_lds = rows && (p.tune.split_lds >= 0 ? p.tune.split_lds == 1 : env_flag("SIP_FFT_SPLIT_LDS", false));
+        a.xcd_remap = p.tune.xcd >= 0 ? p.tune.xcd == 1 : env_flag("SIP_FFT_XCD", false);
         st.tw = pass_twiddles(inst, sign);
         if (io_mode == IO_R2C_EVEN || io_mode == IO_C2R_EVEN) st.rtw = real_roots(L, sign);
         if (tw4_ntot) {
@@ -402,6 +422,7 @@ struct Builder {
             a.tw4_shift = shift;
             a.tw4_mask = (int64_t(1) << shift) - 1;
         }
+        a.col_fixed = ((F * T) % F) == 0 ? 1 : 0;  // always true: block = F * T
         st.grid = dim3((unsigned)((nfft + F - 1) / F));
         st.block = dim3((unsigned)(F * T));
         st.lds = a.split_lds ? (size_t)F * lp * sizeof(float)
@@ -489,7 +510,11 @@ struct Builder {
         }
         // Split L = L1 * L2: L1 column FFTs (+ twiddle), then L2 row FFTs with transposed store.
         int64_t L1 = 0, L2 = 0;
-        if (!leaf_split(L, L1, L2)) {
+        if (depth == 0 && p.tune.split_l1 > 1 && L % p.tune.split_l1 == 0 && p.tune.split_l1 <= kColumnLeafMax &&
+            find_instance(p.tune.split_l1) && decomposable(L / p.tune.split_l1, depth + 1)) {
+            L1 = p.tune.split_l1;
+            L2 = L / L1;
+        } else if (!leaf_split(L, L1, L2)) {
             for (int64_t l1 = std::min(L, kColumnLeafMax); l1 >= 2; l1--)
                 if (L % l1 == 0 && find_instance(l1) && decomposable(L / l1, depth + 1)) {
                     L1 = l1;
@@ -502,7 +527,16 @@ struct Builder {
         // the final store does not go back into that same buffer (stage B is not in-place safe)
         Lay mid = in;
         const bool same_buf = in.buf.kind == out.buf.kind && (in.buf.kind != BUF_WS || in.buf.ws_index == out.buf.ws_index);
-        if (!in_scratch || same_buf) {
+        if (has_table(in.dims)) {
+            // gathered rows: the intermediate is a compact [rows][L] array
+            const int64_t rows = count(normalize(in.dims));
+            mid = Lay{new_ws((size_t)rows * L * 8), 1, {}};
+            int64_t acc = L;
+            for (const Dim &d : in.dims) {
+                mid.dims.push_back({d.n, acc});
+                acc *= d.n;
+            }
+        } else if (!in_scratch || same_buf) {
             int64_t extent = L * in.es;
             for (const Dim &d : in.dims) extent = std::max(extent, d.s * d.n);
             mid.buf = new_ws((size_t)extent * 8);
@@ -702,6 +736,37 @@ void plan_build(Plan &p)
     throw SipError(AsdSip::ACL_ERROR_API_NOT_SUPPORT, "fft type not supported");
 }
 
+void plan_build_gathered_c2c(Plan &p, int64_t n, const std::vector<int64_t> &in_rows,
+                             const std::vector<int64_t> &out_rows, void *in_base, void *out_base)
+{
+    if (in_rows.size() != out_rows.size() || in_rows.empty())
+        throw SipError(AsdSip::ACL_ERROR_INVALID_PARAM, "gathered plan needs matching non-empty row lists");
+    p.stages.clear();
+    p.ws_offsets.clear();
+    p.ws_bytes = 0;
+    p.fftType = kTypeC2C;
+    p.sizes = {n};
+    p.batch = (int64_t)in_rows.size();
+    p.row_tables = {in_rows, out_rows};
+    p.abs_ptrs = {in_base, out_base};
+    Builder b(p);
+    BufRef bi, bo;
+    bi.kind = BUF_ABS;
+    bi.ws_index = 0;
+    bo.kind = BUF_ABS;
+    bo.ws_index = 1;
+    const int64_t rows = (int64_t)in_rows.size();
+    Lay in{bi, 1, {Dim{rows, 0, 0}}};
+    Lay out{bo, 1, {Dim{rows, 0, 1}}};
+    b.axis(n, p.sign(), in, out, false);
+}
+
+Plan::~Plan()
+{
+    for (void *d : row_tables_dev)
+        if (d) (void)hipFree(d);
+}
+
 // ====================================================================== device side
 
 static const void *stage_kernel(const Stage &s)
@@ -713,8 +778,20 @@ static const void *stage_kernel(const Stage &s)
 void plan_materialize(Plan &p)
 {
     SIP_HIP_CHECK(hipGetDevice(&p.device));
+    if (p.row_tables_dev.size() != p.row_tables.size()) {
+        for (void *d : p.row_tables_dev)
+            if (d) (void)hipFree(d);
+        p.row_tables_dev.assign(p.row_tables.size(), nullptr);
+        for (size_t i = 0; i < p.row_tables.size(); i++) {
+            const size_t bytes = p.row_tables[i].size() * sizeof(int64_t);
+            SIP_HIP_CHECK(hipMalloc(&p.row_tables_dev[i], std::max<size_t>(bytes, 8)));
+            SIP_HIP_CHECK(hipMemcpy(p.row_tables_dev[i], p.row_tables[i].data(), bytes, hipMemcpyHostToDevice));
+        }
+    }
     for (Stage &s : p.stages) {
         if (s.kind == StageKind::Stockham) {
+            if (s.in_tab >= 0) s.sa.tin.tab = static_cast<const int64_t *>(p.row_tables_dev[(size_t)s.in_tab]);
+            if (s.out_tab >= 0) s.sa.tout.tab = static_cast<const int64_t *>(p.row_tables_dev[(size_t)s.out_tab]);
             s.tw.dev = coeff_get(s.tw);
             if (s.rtw.used()) s.rtw.dev = coeff_get(s.rtw);
             if (s.tw4lo.used()) {
@@ -741,6 +818,7 @@ static void *resolve(const Plan &p, const BufRef &b, const void *in, void *out)
 {
     if (b.kind == BUF_IN) return const_cast<void *>(in);
     if (b.kind == BUF_OUT) return out;
+    if (b.kind == BUF_ABS) return p.abs_ptrs[(size_t)b.ws_index];
     return static_cast<char *>(p.workspace) + p.ws_offsets[(size_t)b.ws_index];
 }
 

@@ -49,11 +49,20 @@ struct TableRef {
 TablePtr coeff_get(const TableRef &ref);
 size_t coeff_cache_live_tables();
 
-enum BufKind : int { BUF_IN = 0, BUF_OUT = 1, BUF_WS = 2 };
+enum BufKind : int { BUF_IN = 0, BUF_OUT = 1, BUF_WS = 2, BUF_ABS = 3 };
 
 struct BufRef {
     int kind = BUF_IN;
-    int ws_index = 0;  // which workspace region when kind == BUF_WS
+    int ws_index = 0;  // workspace region (BUF_WS) or absolute-pointer slot (BUF_ABS)
+};
+
+// Per-plan tuning choices (0 / -1 = automatic). The autotuner of the batching layer searches them.
+struct Tune {
+    int row_F = 0;         // transforms per workgroup for row launches
+    int split_lds = -1;    // single-plane LDS row kernel
+    int xcd = -1;          // XCD-contiguous tile order
+    int64_t split_l1 = 0;  // forced first factor of a four-step split
+    int col_F = 0;         // transforms per workgroup for column (strided) launches
 };
 
 enum class StageKind { Stockham, Aux };
@@ -68,6 +77,7 @@ struct Stage {
     dim3 grid, block;
     size_t lds = 0;
     TableRef tw, rtw, tw4lo, tw4hi;
+    int in_tab = -1, out_tab = -1;  // row tables (Plan::row_tables) gathered by the in / out tiles
     // Aux
     AuxArgs xa{};
     TableRef tab0;
@@ -94,8 +104,20 @@ struct Plan {
     std::vector<std::string> ref_steps;  // the reference's step list for the same call
     std::mutex mu;
 
+    // Gathered-row plans (batching layer): absolute base pointers and row-offset tables.
+    std::vector<void *> abs_ptrs;
+    std::vector<std::vector<int64_t>> row_tables;
+    std::vector<void *> row_tables_dev;
+    Tune tune;
+
     int sign() const { return direction == kDirInverse ? 1 : -1; }
+    ~Plan();
 };
+
+// 1D C2C plan over rows gathered from arbitrary places: row r of the input starts at element
+// in_rows[r] of in_base, row r of the output at out_rows[r] of out_base (complex elements).
+void plan_build_gathered_c2c(Plan &p, int64_t n, const std::vector<int64_t> &in_rows,
+                             const std::vector<int64_t> &out_rows, void *in_base, void *out_base);
 
 // Lays the plan out (stages, workspace) from its type / sizes / batch. Host only; throws SipError.
 void plan_build(Plan &p);

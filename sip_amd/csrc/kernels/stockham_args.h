@@ -38,6 +38,11 @@ struct TileDesc {
     int32_t unit_f;    // 1 when consecutive transforms are adjacent (column pass): walk f fastest
     int32_t vec2;      // 1 when es == 1 and every row start is 16-byte aligned (float4 accesses)
     FastDiv div_n0, div_n1;
+    // Gathered rows: when tab != null, the offset contributed by index f<tab_dim> is tab[f<tab_dim>]
+    // (elements from the launch's base pointer) instead of f<tab_dim> * s<tab_dim>.
+    const int64_t *tab;
+    int32_t tab_dim;
+    int32_t pad_;
 };
 
 struct StockhamArgs {
@@ -60,6 +65,8 @@ struct StockhamArgs {
     int32_t direct_out;    // last radix pass writes HBM directly (contiguous complex rows)
     int32_t split_lds;     // launched as the single-plane row kernel (direct rows only)
     int32_t xcd_remap;     // give each XCD a contiguous range of tiles
+    int32_t col_fixed;     // column tiles: block size is a multiple of F (one transform per thread)
+    int32_t pad2_;
     FastDiv div_in_len;    // divides by in_len
     FastDiv div_out_len;   // divides by out_len
     FastDiv div_F;         // divides by F

@@ -171,6 +171,21 @@ __device__ __forceinline__ void tile_split(uint32_t f, const TileDesc &d, uint32
     f1 = g - f2 * (uint32_t)d.n1;
 }
 
+// Element offset of transform f's element 0 (strided or gathered through the row table).
+__device__ __forceinline__ int64_t tile_base(uint32_t f, const TileDesc &d, uint32_t *f0_out = nullptr)
+{
+    uint32_t f0, f1, f2;
+    tile_split(f, d, f0, f1, f2);
+    if (f0_out) *f0_out = f0;
+    int64_t o0 = (int64_t)f0 * d.s0, o1 = (int64_t)f1 * d.s1, o2 = (int64_t)f2 * d.s2;
+    if (d.tab) {
+        if (d.tab_dim == 0) o0 = d.tab[f0];
+        else if (d.tab_dim == 1) o1 = d.tab[f1];
+        else o2 = d.tab[f2];
+    }
+    return o0 + o1 + o2;
+}
+
 // W^m with W = e^{s 2 pi i / Ntot}, from the two-level table (m < Ntot).
 __device__ __forceinline__ float2 tw4_root(const StockhamArgs &a, int64_t m)
 {
@@ -266,11 +281,8 @@ __device__ __forceinline__ void split_rows(const StockhamArgs &a, float *pl, int
     const bool active = f < a.nfft;
     int64_t ib = 0, ob = 0;
     if (active) {
-        uint32_t f0, f1, f2;
-        tile_split((uint32_t)f, a.tin, f0, f1, f2);
-        ib = (int64_t)f0 * a.tin.s0 + (int64_t)f1 * a.tin.s1 + (int64_t)f2 * a.tin.s2;
-        tile_split((uint32_t)f, a.tout, f0, f1, f2);
-        ob = (int64_t)f0 * a.tout.s0 + (int64_t)f1 * a.tout.s1 + (int64_t)f2 * a.tout.s2;
+        ib = tile_base((uint32_t)f, a.tin);
+        ob = tile_base((uint32_t)f, a.tout);
     }
     const float2 *g = static_cast<const float2 *>(a.in) + ib;
     float2 v[NB0][R0];
@@ -328,12 +340,10 @@ __global__ __launch_bounds__(1024) void stockham_kernel(const StockhamArgs a)
     // Per-transform base offsets (and the four-step column index).
     for (int fl = tid; fl < nf; fl += nthreads) {
         const uint32_t f = (uint32_t)(fstart + fl);
-        uint32_t f0, f1, f2;
-        tile_split(f, a.tin, f0, f1, f2);
-        in_base[fl] = (int64_t)f0 * a.tin.s0 + (int64_t)f1 * a.tin.s1 + (int64_t)f2 * a.tin.s2;
+        uint32_t f0;
+        in_base[fl] = tile_base(f, a.tin, &f0);
         col[fl] = fdiv(f0, a.div_col);
-        tile_split(f, a.tout, f0, f1, f2);
-        out_base[fl] = (int64_t)f0 * a.tout.s0 + (int64_t)f1 * a.tout.s1 + (int64_t)f2 * a.tout.s2;
+        out_base[fl] = tile_base(f, a.tout);
     }
     __syncthreads();
 
@@ -372,6 +382,20 @@ __global__ __launch_bounds__(1024) void stockham_kernel(const StockhamArgs a)
                 im[fl * LP + lds_pad(j)] = v.y;
                 re[fl * LP + lds_pad(j + 1)] = v.z;
                 im[fl * LP + lds_pad(j + 1)] = v.w;
+            }
+        }
+    } else if (a.tin.unit_f && a.col_fixed) {
+        // column tile: each thread keeps one transform (fl = tid mod F) and walks its elements
+        const int fl = tid - (int)fdiv((uint32_t)tid, a.div_F) * F;
+        const int jstep = nthreads / F;
+        if (fl < nf) {
+            const float2 *p = static_cast<const float2 *>(a.in) + in_base[fl];
+            const int64_t es = a.tin.es;
+            float *pr = re + fl * LP, *pi = im + fl * LP;
+            for (int j = (int)fdiv((uint32_t)tid, a.div_F); j < in_len; j += jstep) {
+                const float2 v = p[(int64_t)j * es];
+                pr[lds_pad(j)] = v.x;
+                pi[lds_pad(j)] = v.y;
             }
         }
     } else {
@@ -511,6 +535,20 @@ __global__ __launch_bounds__(1024) void stockham_kernel(const StockhamArgs a)
                 const float *pr = re + fl * LP, *pi = im + fl * LP;
                 dst[(out_base[fl] >> 1) + jj] =
                     make_float4(pr[lds_pad(j)], pi[lds_pad(j)], pr[lds_pad(j + 1)], pi[lds_pad(j + 1)]);
+            }
+        }
+    } else if (a.tout.unit_f && a.col_fixed) {
+        const int fl = tid - (int)fdiv((uint32_t)tid, a.div_F) * F;
+        const int jstep = nthreads / F;
+        if (fl < nf) {
+            float2 *p = static_cast<float2 *>(a.out) + out_base[fl];
+            const int64_t es = a.tout.es;
+            const float *pr = re + fl * LP, *pi = im + fl * LP;
+            const int64_t c = col[fl];
+            for (int j = (int)fdiv((uint32_t)tid, a.div_F); j < out_len; j += jstep) {
+                float2 v = make_float2(pr[lds_pad(j)], pi[lds_pad(j)]);
+                if (a.tw4_ntot != 0) v = cmul(v, tw4_root(a, c * (int64_t)j));
+                p[(int64_t)j * es] = v;
             }
         }
     } else {

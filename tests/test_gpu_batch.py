@@ -1,0 +1,113 @@
+"""Batching form (sipFftBatch*): uneven C2C problems in one launch.
+
+Checks: every problem against the FP64 oracle (rel-L2 <= 2e-6); the batched result is bit-identical
+to each problem run alone through asdFftExecC2C (same kernels, same per-transform arithmetic);
+hipGraph replays (with eager work and synchronisation in between) reproduce the eager result bit
+for bit; autotuned plans stay within tolerance and reuse the on-disk cache.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import sip_amd as S
+from fftcase import FWD, INV, report
+
+pytestmark = pytest.mark.gpu
+TOL = 2e-6
+
+
+def _problems(torch, lengths, batches, seed):
+    xs, ys, hs = [], [], []
+    for i, (n, b) in enumerate(zip(lengths, batches)):
+        x = oracle.uniform_complex((b, n), seed + i)
+        hs.append(x)
+        xs.append(torch.from_numpy(x).cuda())
+        ys.append(torch.empty((b, n), dtype=torch.complex64, device="cuda"))
+    return hs, xs, ys
+
+
+def _run_batch(torch, lengths, batches, xs, ys, direction, flags):
+    bh = S.BatchHandle()
+    st = bh.make_plan(lengths, batches, [x.data_ptr() for x in xs], [y.data_ptr() for y in ys], direction, flags)
+    assert st == 0, st
+    bh.alloc_workspace()
+    s = torch.cuda.current_stream().cuda_stream
+    if s:
+        assert bh.set_stream(s) == 0
+    assert bh.exec() == 0
+    torch.cuda.synchronize()
+    return bh
+
+
+LENGTHS = [64, 100, 4096, 4096, 1536, 2000, 8192, 16384, 20000, 65536, 3000, 128, 64, 59049 // 243 * 243, 25,
+           30000, 45, 1024, 1000, 36]
+BATCHES = [3, 1, 2, 7, 5, 4, 2, 1, 2, 1, 6, 9, 2, 1, 8, 1, 3, 4, 2, 5]
+
+
+@pytest.mark.parametrize("direction", [FWD, INV])
+def test_batch_matches_oracle_and_alone(torch_cuda, direction):
+    torch = torch_cuda
+    hs, xs, ys = _problems(torch, LENGTHS, BATCHES, 100)
+    bh = _run_batch(torch, LENGTHS, BATCHES, xs, ys, direction, S.BATCH_GRAPH)
+    d = bh.describe()
+    assert d["groups"] == len(set(LENGTHS)) and d["graph"] == 1
+    for i, (n, b) in enumerate(zip(LENGTHS, BATCHES)):
+        got = ys[i].cpu().numpy()
+        rel, _ = report(f"batch problem {i} n={n} b={b}", got, oracle.c2c(hs[i], oracle.FORWARD if direction == FWD
+                                                                          else oracle.INVERSE))
+        assert rel <= TOL
+        # alone: the same problem through a plain plan must give the same bits
+        h = S.FftHandle()
+        assert h.make_plan_1d(n, S.ASCEND_FFT_C2C, direction, b) == 0
+        h.alloc_workspace()
+        h.use_current_torch_stream()
+        alone = torch.empty_like(ys[i])
+        assert h.exec_c2c(xs[i], alone) == 0
+        assert h.synchronize() == 0
+        assert torch.equal(alone, ys[i]), f"problem {i}: batched result differs from the problem alone"
+        h.destroy()
+    bh.destroy()
+
+
+def test_graph_replays_match_eager(torch_cuda):
+    torch = torch_cuda
+    hs, xs, ys = _problems(torch, LENGTHS, BATCHES, 300)
+    eager_ys = [torch.empty_like(y) for y in ys]
+    be = _run_batch(torch, LENGTHS, BATCHES, xs, eager_ys, FWD, 0)          # direct launches
+    bg = _run_batch(torch, LENGTHS, BATCHES, xs, ys, FWD, S.BATCH_GRAPH)    # graph
+    for rep in range(4):
+        for y in ys:
+            y.zero_()
+        junk = torch.randn(1 << 20, device="cuda") * 3  # eager work between replays
+        torch.cuda.synchronize()
+        assert bg.exec() == 0
+        torch.cuda.synchronize()
+        for i in range(len(ys)):
+            assert torch.equal(ys[i], eager_ys[i]), f"replay {rep} problem {i}"
+        del junk
+    be.destroy()
+    bg.destroy()
+
+
+def test_autotune_and_cache(torch_cuda, tmp_path, monkeypatch):
+    torch = torch_cuda
+    monkeypatch.setenv("SIP_FFT_TUNE_CACHE", str(tmp_path / "tune.jsonl"))
+    lengths = [4096, 1024, 65536, 1536, 20000]
+    batches = [64, 256, 4, 32, 3]
+    hs, xs, ys = _problems(torch, lengths, batches, 500)
+    bh = _run_batch(torch, lengths, batches, xs, ys, FWD, S.BATCH_GRAPH | S.BATCH_AUTOTUNE)
+    d = bh.describe()
+    assert all(g["tuned_us"] > 0 for g in d["group_list"]), d
+    for i in range(len(lengths)):
+        rel, _ = report(f"tuned problem {i}", ys[i].cpu().numpy(), oracle.c2c(hs[i]))
+        assert rel <= TOL
+    bh.destroy()
+    lines = (tmp_path / "tune.jsonl").read_text().strip().splitlines()
+    assert len(lines) == len(lengths)
+
+
+def test_batching_workload_shape():
+    lengths, batches = S.batching_workload(4096)
+    sm = set(S.smooth_lengths(64, 65536))
+    assert len(lengths) == 4096 and all(n in sm for n in lengths) and all(1 <= b <= 64 for b in batches)
+    assert lengths == S.batching_workload(4096)[0]

@@ -63,7 +63,7 @@ def test_c2_r2c_c2r_8192_b4096(torch_cuda):
 
 POW2 = [2 ** k for k in range(1, 15)]
 SMOOTH = [3, 5, 6, 9, 10, 12, 15, 25, 27, 45, 60, 81, 100, 125, 243, 360, 625, 768, 1000, 1080, 2187, 2400,
-          3000, 3125, 3840, 4000]
+          3000, 3125, 3840, 4000, 4374, 5120, 6000, 9000, 12288, 13122, 15360, 16000]
 PRIME_FACTOR = [7, 11, 13, 14, 21, 49, 77, 91, 143, 169, 343, 1001, 1331, 2197, 4004]
 
 
