@@ -93,6 +93,27 @@ int32_t sipFftBatchExec(void *handle);
 int32_t sipFftBatchDescribe(void *handle, char *buf, size_t cap, size_t *needed);
 int32_t sipFftBatchDestroy(void *handle);
 
+/* Serving form: a continuous-batching request loop (one scheduler thread per server).
+ * Create: budget_bytes of device memory are reserved once as the admission arena (outputs owned by
+ * the server, workspaces, row tables); nstreams streams carry up to two launches each; queued
+ * requests of the same (type, n) are coalesced into one launch of at most max_coalesce_rows rows.
+ * Submit: fftType C2C (0x10, complex64 in/out) or R2C (0x12, float in, n/2+1 bins out), forward;
+ * input is a device address; output a device address, or NULL to let the server allocate it from the
+ * arena (the result is then discarded after completion). Wait blocks until the request finished and
+ * returns its status and latency (arrival to observed completion). Stats returns JSON
+ * (p50/p90/p99 latency, launches, coalescing, arena use). Benchmark runs an open-loop Poisson
+ * stream (see serving.cpp). Calls that overlap on one server are serialised internally. */
+int32_t sipFftServeCreate(void **handle, size_t budget_bytes, int32_t nstreams, int64_t max_coalesce_rows);
+int32_t sipFftServeSubmit(void *handle, int32_t fftType, int64_t n, int64_t batch, const void *input, void *output,
+                          uint64_t *ticket);
+int32_t sipFftServeWait(void *handle, uint64_t ticket, double *latency_ms);
+int32_t sipFftServeDrain(void *handle);
+int32_t sipFftServeStats(void *handle, int32_t reset, char *buf, size_t cap, size_t *needed);
+int32_t sipFftServeBenchmark(void *handle, double rate, double seconds, uint64_t seed, const void *pool,
+                             size_t pool_bytes, double c2c_frac, int64_t n_lo, int64_t n_hi, int64_t b_max,
+                             double drain_seconds, char *buf, size_t cap, size_t *needed);
+int32_t sipFftServeDestroy(void *handle);
+
 #ifdef __cplusplus
 }
 #endif

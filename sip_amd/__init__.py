@@ -52,6 +52,8 @@ EXPORTED_SYMBOLS = [
     "aclFreeViewShape", "aclGetTensorData", "sipFftPlanDescribe", "sipFftPlanDescribeDry", "sipFftVersion",
     "sipFftBatchCreate", "sipFftBatchSetStream", "sipFftBatchMakePlan", "sipFftBatchGetWorkspaceSize",
     "sipFftBatchSetWorkspace", "sipFftBatchExec", "sipFftBatchDescribe", "sipFftBatchDestroy",
+    "sipFftServeCreate", "sipFftServeSubmit", "sipFftServeWait", "sipFftServeDrain", "sipFftServeStats",
+    "sipFftServeBenchmark", "sipFftServeDestroy",
 ]
 
 BATCH_GRAPH = 1
@@ -104,6 +106,14 @@ def lib():
         "sipFftBatchExec": [vp],
         "sipFftBatchDescribe": [vp, ctypes.c_char_p, sz, P(sz)],
         "sipFftBatchDestroy": [vp],
+        "sipFftServeCreate": [P(vp), sz, i32, i64],
+        "sipFftServeSubmit": [vp, i32, i64, i64, vp, vp, P(ctypes.c_uint64)],
+        "sipFftServeWait": [vp, ctypes.c_uint64, P(ctypes.c_double)],
+        "sipFftServeDrain": [vp],
+        "sipFftServeStats": [vp, i32, ctypes.c_char_p, sz, P(sz)],
+        "sipFftServeBenchmark": [vp, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, vp, sz, ctypes.c_double,
+                                 i64, i64, i64, ctypes.c_double, ctypes.c_char_p, sz, P(sz)],
+        "sipFftServeDestroy": [vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -367,3 +377,50 @@ def batching_workload(count=4096, seed=0x5EED0004, lo=64, hi=65536, max_batch=64
         lengths.append(int(n))
         batches.append(int(rng.integers(1, max_batch + 1)))
     return lengths, batches
+
+
+class Server:
+    """Serving form (sipFftServe*): continuous-batching FFT request loop on one GPU."""
+
+    def __init__(self, budget_bytes=64 << 30, nstreams=4, max_coalesce_rows=8192):
+        h = ctypes.c_void_p()
+        _check(lib().sipFftServeCreate(ctypes.byref(h), int(budget_bytes), nstreams, max_coalesce_rows),
+               "sipFftServeCreate")
+        self.h = h
+
+    def submit(self, fft_type, n, batch, in_ptr, out_ptr=None):
+        t = ctypes.c_uint64()
+        _check(lib().sipFftServeSubmit(self.h, fft_type, n, batch, ctypes.c_void_p(int(in_ptr)),
+                                       ctypes.c_void_p(int(out_ptr) if out_ptr else None), ctypes.byref(t)),
+               "sipFftServeSubmit")
+        return t.value
+
+    def wait(self, ticket):
+        lat = ctypes.c_double()
+        st = lib().sipFftServeWait(self.h, ticket, ctypes.byref(lat))
+        return st, lat.value
+
+    def drain(self):
+        return lib().sipFftServeDrain(self.h)
+
+    def stats(self, reset=False) -> dict:
+        return json.loads(_describe(lambda buf, cap, need: lib().sipFftServeStats(self.h, int(reset), buf, cap, need)))
+
+    def benchmark(self, rate, seconds, pool_ptr, pool_bytes, seed=1, c2c_frac=0.7, n_lo=256, n_hi=32768, b_max=512,
+                  drain_seconds=30.0) -> dict:
+        return json.loads(_describe(lambda buf, cap, need: lib().sipFftServeBenchmark(
+            self.h, rate, seconds, seed, ctypes.c_void_p(int(pool_ptr)), int(pool_bytes), c2c_frac, n_lo, n_hi,
+            b_max, drain_seconds, buf, cap, need)))
+
+    def destroy(self):
+        if self.h is not None:
+            st = lib().sipFftServeDestroy(self.h)
+            self.h = None
+            return st
+        return ACL_SUCCESS
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -739,15 +739,32 @@ void plan_build(Plan &p)
 void plan_build_gathered_c2c(Plan &p, int64_t n, const std::vector<int64_t> &in_rows,
                              const std::vector<int64_t> &out_rows, void *in_base, void *out_base)
 {
+    plan_build_gathered(p, kTypeC2C, n, in_rows, out_rows, in_base, out_base);
+}
+
+bool gathered_supported(int type, int64_t n)
+{
+    if (type == kTypeC2C) return n >= 1;
+    if (type == kTypeR2C) {
+        if (n % 2 == 0 && n >= 4) return find_instance(n / 2) && n / 2 <= kRowLeafMax;
+        return n >= 3 && find_instance(n) && n <= kRowLeafMax;
+    }
+    return false;
+}
+
+void plan_build_gathered(Plan &p, int type, int64_t n, const std::vector<int64_t> &in_rows,
+                         const std::vector<int64_t> &out_rows, void *in_base, void *out_base)
+{
     if (in_rows.size() != out_rows.size() || in_rows.empty())
         throw SipError(AsdSip::ACL_ERROR_INVALID_PARAM, "gathered plan needs matching non-empty row lists");
+    if (!gathered_supported(type, n))
+        throw SipError(AsdSip::ACL_ERROR_API_NOT_SUPPORT, "gathered rows not supported for this type / length");
     p.stages.clear();
     p.ws_offsets.clear();
     p.ws_bytes = 0;
-    p.fftType = kTypeC2C;
+    p.fftType = type;
     p.sizes = {n};
     p.batch = (int64_t)in_rows.size();
-    p.row_tables = {in_rows, out_rows};
     p.abs_ptrs = {in_base, out_base};
     Builder b(p);
     BufRef bi, bo;
@@ -758,7 +775,25 @@ void plan_build_gathered_c2c(Plan &p, int64_t n, const std::vector<int64_t> &in_
     const int64_t rows = (int64_t)in_rows.size();
     Lay in{bi, 1, {Dim{rows, 0, 0}}};
     Lay out{bo, 1, {Dim{rows, 0, 1}}};
-    b.axis(n, p.sign(), in, out, false);
+    if (type == kTypeC2C) {
+        p.row_tables = {in_rows, out_rows};
+        b.axis(n, p.sign(), in, out, false);
+        return;
+    }
+    // R2C: even n reads each real row as n/2 complex values (row offsets given in floats)
+    const int64_t h = n / 2 + 1;
+    if (n % 2 == 0) {
+        std::vector<int64_t> cin(in_rows.size());
+        for (size_t i = 0; i < in_rows.size(); i++) {
+            if (in_rows[i] % 2) throw SipError(AsdSip::ACL_ERROR_INVALID_PARAM, "R2C rows must start at even floats");
+            cin[i] = in_rows[i] / 2;
+        }
+        p.row_tables = {cin, out_rows};
+        b.add_stockham(n / 2, p.sign(), in, out, IO_R2C_EVEN, n / 2, h, 0, 1, "gathered r2c even " + std::to_string(n));
+    } else {
+        p.row_tables = {in_rows, out_rows};
+        b.add_stockham(n, p.sign(), in, out, IO_R2C_FULL, n, h, 0, 1, "gathered r2c full " + std::to_string(n));
+    }
 }
 
 Plan::~Plan()
@@ -775,10 +810,24 @@ static const void *stage_kernel(const Stage &s)
     return s.sign < 0 ? s.inst->fwd : s.inst->inv;
 }
 
+void plan_materialize_external(Plan &p, const std::vector<const int64_t *> &dev_tables)
+{
+    // row tables already on the device (caller-owned, e.g. uploaded asynchronously)
+    p.row_tables_dev.clear();
+    p.external_tables = dev_tables;
+    plan_materialize(p);
+}
+
 void plan_materialize(Plan &p)
 {
     SIP_HIP_CHECK(hipGetDevice(&p.device));
-    if (p.row_tables_dev.size() != p.row_tables.size()) {
+    if (!p.external_tables.empty()) {
+        for (Stage &s : p.stages) {
+            if (s.kind != StageKind::Stockham) continue;
+            if (s.in_tab >= 0) s.sa.tin.tab = p.external_tables[(size_t)s.in_tab];
+            if (s.out_tab >= 0) s.sa.tout.tab = p.external_tables[(size_t)s.out_tab];
+        }
+    } else if (p.row_tables_dev.size() != p.row_tables.size()) {
         for (void *d : p.row_tables_dev)
             if (d) (void)hipFree(d);
         p.row_tables_dev.assign(p.row_tables.size(), nullptr);
@@ -790,8 +839,10 @@ void plan_materialize(Plan &p)
     }
     for (Stage &s : p.stages) {
         if (s.kind == StageKind::Stockham) {
-            if (s.in_tab >= 0) s.sa.tin.tab = static_cast<const int64_t *>(p.row_tables_dev[(size_t)s.in_tab]);
-            if (s.out_tab >= 0) s.sa.tout.tab = static_cast<const int64_t *>(p.row_tables_dev[(size_t)s.out_tab]);
+            if (p.external_tables.empty()) {
+                if (s.in_tab >= 0) s.sa.tin.tab = static_cast<const int64_t *>(p.row_tables_dev[(size_t)s.in_tab]);
+                if (s.out_tab >= 0) s.sa.tout.tab = static_cast<const int64_t *>(p.row_tables_dev[(size_t)s.out_tab]);
+            }
             s.tw.dev = coeff_get(s.tw);
             if (s.rtw.used()) s.rtw.dev = coeff_get(s.rtw);
             if (s.tw4lo.used()) {

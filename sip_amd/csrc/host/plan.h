@@ -108,16 +108,24 @@ struct Plan {
     std::vector<void *> abs_ptrs;
     std::vector<std::vector<int64_t>> row_tables;
     std::vector<void *> row_tables_dev;
+    std::vector<const int64_t *> external_tables;  // caller-owned device row tables (serving layer)
     Tune tune;
 
     int sign() const { return direction == kDirInverse ? 1 : -1; }
     ~Plan();
 };
 
-// 1D C2C plan over rows gathered from arbitrary places: row r of the input starts at element
-// in_rows[r] of in_base, row r of the output at out_rows[r] of out_base (complex elements).
+// 1D plan over rows gathered from arbitrary places: row r of the input starts at element
+// in_rows[r] of in_base, row r of the output at out_rows[r] of out_base (elements of each buffer's
+// own type: complex for C2C and R2C outputs, float for R2C inputs). C2C: any length; R2C: lengths
+// whose (half) transform is one leaf launch (gathered_supported).
+void plan_build_gathered(Plan &p, int type, int64_t n, const std::vector<int64_t> &in_rows,
+                         const std::vector<int64_t> &out_rows, void *in_base, void *out_base);
 void plan_build_gathered_c2c(Plan &p, int64_t n, const std::vector<int64_t> &in_rows,
                              const std::vector<int64_t> &out_rows, void *in_base, void *out_base);
+bool gathered_supported(int type, int64_t n);
+// plan_materialize with the row tables already on the device (one pointer per Plan::row_tables).
+void plan_materialize_external(Plan &p, const std::vector<const int64_t *> &dev_tables);
 
 // Lays the plan out (stages, workspace) from its type / sizes / batch. Host only; throws SipError.
 void plan_build(Plan &p);
