@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-batching", action="store_true", help="skip the uneven-batch (batching form) leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-form leg")
+    ap.add_argument("--no-precision", action="store_true", help="skip the precision-form (MFMA) leg")
     ap.add_argument("--serving-seconds", type=float, default=2.0,
                     help="arrival window per rate (the serving configuration specifies 60 s)")
     return ap.parse_args()
@@ -64,12 +65,16 @@ def timed_exec(torch, h, kind, x, y, steps, warmup, stream):
     return wall, dev_ms / steps
 
 
-def make_handle(S, torch, dims, fft_type, batch, stream):
+def make_handle(S, torch, dims, fft_type, batch, stream, precision=0):
     h = S.FftHandle()
+    if precision:
+        assert h.set_precision(precision) == 0
     if len(dims) == 1:
         st = h.make_plan_1d(dims[0], fft_type, S.ASCEND_FFT_FORWARD, batch)
-    else:
+    elif len(dims) == 2:
         st = h.make_plan_2d(dims[0], dims[1], fft_type, S.ASCEND_FFT_FORWARD, batch)
+    else:
+        st = h.make_plan_3d(dims[0], dims[1], dims[2], fft_type, S.ASCEND_FFT_FORWARD, batch)
     assert st == 0, st
     h.alloc_workspace()
     assert h.set_stream(stream.cuda_stream) == 0
@@ -157,6 +162,48 @@ def bench_batching(torch, S, stream, reps=5):
     res["plan_and_tune_s"] = round(tune_s, 2)
     del xin, yout
     torch.cuda.empty_cache()
+    return res
+
+
+def bench_precision(torch, S, stream, steps):
+    """Precision form next to the plain form: the same C2C plans with the radix-16 stages on the
+    matrix cores (FP16 hi+lo, BF16 hi+lo, BF16 hi+mid+lo), GFLOP/s (5 N log2 N) and device time per
+    exec; for the 1D headline shape also rel-L2 vs the FP64 oracle on 64 sampled rows."""
+    import numpy as np
+    import oracle
+    modes = [("fp32", 0), ("f16x2", S.PREC_F16_SPLIT), ("bf16x2", S.PREC_BF16_SPLIT),
+             ("bf16x3", S.PREC_BF16_SPLIT3)]
+    cases = [("c2c_4096x16384", [4096], 16384), ("c2c_2d_1024x1024x8", [1024, 1024], 8),
+             ("c2c_3d_128x128x128x4", [128, 128, 128], 4)]
+    res = {}
+    for name, dims, b in cases:
+        nn = int(math.prod(dims))
+        with torch.cuda.stream(stream):
+            g = torch.Generator(device="cuda")
+            g.manual_seed(0x5EED0005)
+            xi = torch.view_as_complex(torch.rand([b] + dims + [2], generator=g, device="cuda") * 2 - 1).contiguous()
+            yo = torch.empty_like(xi)
+        torch.cuda.synchronize()
+        fl = 5.0 * nn * math.log2(nn) * b
+        rows = None
+        if len(dims) == 1:
+            rows = np.arange(0, b, b // 64)
+            xs = xi.cpu().numpy()[rows]
+            ref = oracle.c2c(xs)
+        entry = {}
+        for mname, mode in modes:
+            hh = make_handle(S, torch, dims, S.ASCEND_FFT_C2C, b, stream, mode)
+            st = max(10, steps // 2)
+            w, kms = timed_exec(torch, hh, "c2c", xi, yo, st, 3, stream)
+            e = {"GFLOP/s": round(fl * st / w / 1e9, 1), "device_ms": round(kms, 5),
+                 "device_GFLOP/s": round(fl / (kms * 1e-3) / 1e9, 1)}
+            if rows is not None:
+                rel, mx = oracle.error(yo.cpu().numpy()[rows], ref)
+                e["rel_l2_sample"] = float(f"{rel:.3e}")
+            entry[mname] = e
+            hh.destroy()
+        res[name] = entry
+        del xi, yo
     return res
 
 
@@ -301,6 +348,9 @@ def main():
             hh.destroy()
             del xi, yo
         out["secondary"] = sec
+
+    if not args.no_precision:
+        out["precision"] = bench_precision(torch, S, stream, args.steps)
 
     if not args.no_batching:
         out["batching"] = bench_batching(torch, S, stream)

@@ -76,6 +76,13 @@ int32_t sipFftPlanDescribeDry(int32_t ndim, const int64_t *dims, int32_t fftType
 /* Library version string. */
 const char *sipFftVersion(void);
 
+/* Precision form: run the radix-16 stages of C2C launches on the matrix cores with split operands
+ * and FP32 accumulation. mode 0 = plain FP32 (default), 1 = FP16 hi+lo (per-column power-of-two
+ * scaling), 2 = BF16 hi+lo, 3 = BF16 hi+mid+lo. Applies to leaf lengths 128, 256, 512, 1024, 2048
+ * and 4096 (other launches stay plain). Call before or after MakePlan (the plan is rebuilt); the
+ * workspace size does not change. */
+int32_t sipFftSetPrecision(void *handle, int32_t mode);
+
 /* Batching form: `count` C2C problems of uneven lengths run as one launch. Problem i is
  * batches[i] transforms of length lengths[i], contiguous complex64 at device address in_ptrs[i]
  * (8-byte aligned), written to out_ptrs[i]. The buffers are bound at plan time.

@@ -53,8 +53,10 @@ EXPORTED_SYMBOLS = [
     "sipFftBatchCreate", "sipFftBatchSetStream", "sipFftBatchMakePlan", "sipFftBatchGetWorkspaceSize",
     "sipFftBatchSetWorkspace", "sipFftBatchExec", "sipFftBatchDescribe", "sipFftBatchDestroy",
     "sipFftServeCreate", "sipFftServeSubmit", "sipFftServeWait", "sipFftServeDrain", "sipFftServeStats",
-    "sipFftServeBenchmark", "sipFftServeDestroy",
+    "sipFftServeBenchmark", "sipFftServeDestroy", "sipFftSetPrecision",
 ]
+
+PREC_FP32, PREC_F16_SPLIT, PREC_BF16_SPLIT, PREC_BF16_SPLIT3 = 0, 1, 2, 3
 
 BATCH_GRAPH = 1
 BATCH_AUTOTUNE = 2
@@ -114,6 +116,7 @@ def lib():
         "sipFftServeBenchmark": [vp, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, vp, sz, ctypes.c_double,
                                  i64, i64, i64, ctypes.c_double, ctypes.c_char_p, sz, P(sz)],
         "sipFftServeDestroy": [vp],
+        "sipFftSetPrecision": [vp, i32],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -205,6 +208,9 @@ class FftHandle:
 
     def make_plan_3d(self, x, y, z, fft_type, direction, batch):
         return lib().asdFftMakePlan3D(self.h, x, y, z, fft_type, direction, batch)
+
+    def set_precision(self, mode):
+        return lib().sipFftSetPrecision(self.h, mode)
 
     def workspace_size(self) -> int:
         n = ctypes.c_size_t()

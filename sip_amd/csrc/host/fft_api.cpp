@@ -408,4 +408,16 @@ int32_t sipFftPlanDescribeDry(int32_t ndim, const int64_t *dims, int32_t fftType
 
 const char *sipFftVersion(void) { return "sip_fft 0.1 (gfx950)"; }
 
+int32_t sipFftSetPrecision(void *handle, int32_t mode)
+{
+    return guarded([&]() -> int32_t {
+        Plan *p = lookup(handle);
+        if (!p || mode < sip::PREC_FP32 || mode > sip::PREC_BF16_SPLIT3) return E::ACL_ERROR_INVALID_PARAM;
+        std::lock_guard<std::mutex> lk(p->mu);
+        p->precision = mode;
+        if (p->initialized) return finish_plan(*p);  // re-plan with the new stage kernels
+        return E::ACL_SUCCESS;
+    });
+}
+
 }  // extern "C"

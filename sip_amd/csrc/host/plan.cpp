@@ -346,6 +346,10 @@ struct Builder {
                       int64_t out_len, int64_t tw4_ntot, int64_t col_div, const std::string &note)
     {
         const StockhamInstance *inst = find_instance(L);
+        // precision form: C2C launches with radix-16 stages run them on the matrix cores
+        const StockhamInstance *mf = (p.precision != PREC_FP32 && io_mode == IO_C2C)
+                                         ? find_mfma_instance(L, p.precision) : nullptr;
+        if (mf) inst = mf;
         if (!inst) throw SipError(AsdSip::ACL_ERROR_API_NOT_SUPPORT, "no kernel for length " + std::to_string(L));
         bool ok = true;
         Stage st;
@@ -389,8 +393,16 @@ struct Builder {
             }
             if (f >= 1 && f * T <= 1024) F = f;
         }
-        if (F > nfft) F = (int)nfft;
-        if (F < 1) F = 1;
+        if (mf) {
+            // every wave owns 4 blocks of 16 butterflies: F * T must be a multiple of 64
+            while ((F * T) % 64 != 0) F++;
+            while (F > 1 && ((size_t)F * per > 80 * 1024 || F * T > 1024) && ((F / 2) * T) % 64 == 0) F /= 2;
+            if (F * T > 1024 || (size_t)F * per > 160 * 1024)
+                throw SipError(AsdSip::ACL_ERROR_API_NOT_SUPPORT, "no MFMA tile fits length " + std::to_string(L));
+        } else {
+            if (F > nfft) F = (int)nfft;
+            if (F < 1) F = 1;
+        }
         a.F = F;
         a.div_in_len = make_fastdiv((uint32_t)in_len);
         a.div_out_len = make_fastdiv((uint32_t)out_len);
@@ -410,10 +422,12 @@ struct Builder {
         // Measured on MI355X (N=4096 x 16384): the two-plane kernel in launch order beats the
         // single-plane exchange kernel (twice the resident workgroups) and XCD-contiguous tile order;
         // both stay available as tuning switches SIP_FFT_SPLIT_LDS=1 and SIP_FFT_XCD=1.
+        if (mf) a.direct_in = a.direct_out = 0;  // MFMA tiles are LDS-staged
         const bool rows = io_mode == IO_C2C && a.direct_in && a.direct_out && inst->rows_fwd != nullptr;
         a.split_lds = rows && (p.tune.split_lds >= 0 ? p.tune.split_lds == 1 : env_flag("SIP_FFT_SPLIT_LDS", false));
         a.xcd_remap = p.tune.xcd >= 0 ? p.tune.xcd == 1 : env_flag("SIP_FFT_XCD", false);
         st.tw = pass_twiddles(inst, sign);
+        if (mf) st.mfw = mfma_dft_table(p.precision, sign);
         if (io_mode == IO_R2C_EVEN || io_mode == IO_C2R_EVEN) st.rtw = real_roots(L, sign);
         if (tw4_ntot) {
             four_step_tables(tw4_ntot, sign, st.tw4lo, st.tw4hi);
@@ -850,6 +864,10 @@ void plan_materialize(Plan &p)
                 s.tw4hi.dev = coeff_get(s.tw4hi);
             }
             s.sa.tw = static_cast<const float2 *>(s.tw.dev->ptr);
+            if (s.mfw.used()) {
+                s.mfw.dev = coeff_get(s.mfw);
+                s.sa.mfma_w = static_cast<const unsigned short *>(s.mfw.dev->ptr);
+            }
             s.sa.rtw = s.rtw.dev ? static_cast<const float2 *>(s.rtw.dev->ptr) : nullptr;
             s.sa.tw4_lo = s.tw4lo.dev ? static_cast<const float2 *>(s.tw4lo.dev->ptr) : nullptr;
             s.sa.tw4_hi = s.tw4hi.dev ? static_cast<const float2 *>(s.tw4hi.dev->ptr) : nullptr;
@@ -922,7 +940,7 @@ std::string plan_describe(const Plan &p)
                << ",\"grid\":" << s.grid.x << ",\"block\":" << s.block.x << ",\"unit_f_in\":" << s.sa.tin.unit_f
                << ",\"unit_f_out\":" << s.sa.tout.unit_f << ",\"tw4\":" << s.sa.tw4_ntot
                << ",\"direct_in\":" << s.sa.direct_in << ",\"direct_out\":" << s.sa.direct_out
-               << ",\"split_lds\":" << s.sa.split_lds
+               << ",\"split_lds\":" << s.sa.split_lds << ",\"mfma\":" << (s.mfw.used() ? 1 : 0)
                << ",\"xcd_remap\":" << s.sa.xcd_remap;
         } else {
             os << ",\"mode\":" << s.xa.mode << ",\"rows\":" << s.xa.rows << ",\"n\":" << s.xa.n

@@ -76,7 +76,7 @@ struct Stage {
     StockhamArgs sa{};
     dim3 grid, block;
     size_t lds = 0;
-    TableRef tw, rtw, tw4lo, tw4hi;
+    TableRef tw, rtw, tw4lo, tw4hi, mfw;
     int in_tab = -1, out_tab = -1;  // row tables (Plan::row_tables) gathered by the in / out tiles
     // Aux
     AuxArgs xa{};
@@ -110,6 +110,7 @@ struct Plan {
     std::vector<void *> row_tables_dev;
     std::vector<const int64_t *> external_tables;  // caller-owned device row tables (serving layer)
     Tune tune;
+    int precision = 0;  // PrecisionMode of the radix-16 stages of C2C launches
 
     int sign() const { return direction == kDirInverse ? 1 : -1; }
     ~Plan();
@@ -138,6 +139,9 @@ std::string plan_describe(const Plan &p);
 
 // Kernel registry (generated instances).
 const StockhamInstance *find_instance(int64_t L);
+// Split-precision MFMA instances (mode = PrecisionMode) and their DFT-matrix operand table.
+const StockhamInstance *find_mfma_instance(int64_t L, int mode);
+TableRef mfma_dft_table(int mode, int sign);
 int64_t instance_count();
 
 // Launchers implemented in the .hip files.

@@ -1,0 +1,252 @@
+// Split-precision radix-16 Stockham stage on the matrix cores (precision form).
+//
+// A radix-16 stage maps 16 complex inputs of every butterfly to 16 outputs with the DFT matrix
+// W[q][r] = e^{s 2 pi i q r / 16}. In real form that is one GEMM per block of 16 butterflies:
+//     [Y_re; Y_im] (32 x 16) = [[W_re, -W_im], [W_im, W_re]] (32 x 32) . [X_re; X_im] (32 x 16),
+// run as 2 x 2 v_mfma_f32_16x16x16 tiles (A = DFT matrix, B = butterfly columns, FP32 accumulate).
+// FP16 / BF16 operands lose precision, so both operands are split: x = x_hi + x_lo (+ x_mid for
+// three-term BF16), w likewise (split in FP64 on the host), and the products that matter are
+// summed: hi*hi + hi*lo + lo*hi (two terms) or hi*hi + hi*mid + mid*hi + hi*lo + lo*hi + mid*mid
+// (three terms), which brings the stage back to about FP32 accuracy. FP16 has a narrow range: each
+// butterfly column is scaled by a power of two (max |x| -> [2^13, 2^14)) before the split and
+// unscaled after the GEMM, exactly.
+//
+// Lane layout of v_mfma_f32_16x16x16 (f16 and bf16_1k), verified on gfx950: lane l holds
+// A[l % 16][4 (l / 16) + t], B[4 (l / 16) + t][l % 16] and D[4 (l / 16) + t][l % 16], t = 0..3.
+// With T = L / 16 threads per transform each wave owns exactly 4 blocks of 16 butterflies.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "stockham_device.h"
+
+namespace sip {
+
+typedef _Float16 mf_half4 __attribute__((ext_vector_type(4)));
+typedef short mf_short4 __attribute__((ext_vector_type(4)));
+typedef float mf_float4 __attribute__((ext_vector_type(4)));
+
+enum MfmaDtype { MF_F16 = 1, MF_BF16 = 2 };
+
+template <int DT>
+struct MfOp;
+template <>
+struct MfOp<MF_F16> {
+    typedef mf_half4 vec;
+    static __device__ __forceinline__ unsigned short to_bits(float x)
+    {
+        const _Float16 h = (_Float16)x;
+        return __builtin_bit_cast(unsigned short, h);
+    }
+    static __device__ __forceinline__ float from_bits(unsigned short b)
+    {
+        return (float)__builtin_bit_cast(_Float16, b);
+    }
+    static __device__ __forceinline__ mf_float4 mma(vec a, vec b, mf_float4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ vec pack(const unsigned short *v)
+    {
+        vec r;
+#pragma unroll
+        for (int t = 0; t < 4; t++) r[t] = __builtin_bit_cast(_Float16, v[t]);
+        return r;
+    }
+};
+template <>
+struct MfOp<MF_BF16> {
+    typedef mf_short4 vec;
+    static __device__ __forceinline__ unsigned short to_bits(float x)
+    {
+        // round to nearest even (finite inputs)
+        const unsigned u = __float_as_uint(x);
+        return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+    }
+    static __device__ __forceinline__ float from_bits(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+    static __device__ __forceinline__ mf_float4 mma(vec a, vec b, mf_float4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ vec pack(const unsigned short *v)
+    {
+        vec r;
+#pragma unroll
+        for (int t = 0; t < 4; t++) r[t] = (short)v[t];
+        return r;
+    }
+};
+
+// DFT-matrix operands of one lane: [term][rb][kb] (A fragments), loaded once per kernel from the
+// host-built table laid out [term][rb][kb][lane][4] (16-bit values).
+template <int DT, int NT>
+struct MfW {
+    typename MfOp<DT>::vec a[NT][2][2];
+    __device__ __forceinline__ void load(const unsigned short *__restrict__ tab, int lane)
+    {
+#pragma unroll
+        for (int tm = 0; tm < NT; tm++)
+#pragma unroll
+            for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                for (int kb = 0; kb < 2; kb++) {
+                    const unsigned short *p = tab + ((((tm * 2 + rb) * 2 + kb) * 64 + lane) * 4);
+                    unsigned short v[4];
+#pragma unroll
+                    for (int t = 0; t < 4; t++) v[t] = p[t];
+                    a[tm][rb][kb] = MfOp<DT>::pack(v);
+                }
+    }
+};
+
+// Splits the 4 values of a lane's column slice into NT terms (optionally after scaling).
+template <int DT, int NT>
+__device__ __forceinline__ void mf_split(const float (&x)[4], float scale, typename MfOp<DT>::vec (&out)[NT])
+{
+    unsigned short b[NT][4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        float r = x[t] * scale;
+#pragma unroll
+        for (int tm = 0; tm < NT; tm++) {
+            b[tm][t] = MfOp<DT>::to_bits(r);
+            r -= MfOp<DT>::from_bits(b[tm][t]);
+        }
+    }
+#pragma unroll
+    for (int tm = 0; tm < NT; tm++) out[tm] = MfOp<DT>::pack(b[tm]);
+}
+
+// One radix-16 pass on MFMA for a tile of F transforms of length L in LDS (T = L/16 threads per
+// transform, F * T a multiple of 64). NS = product of the earlier radices.
+template <int L, int T, int S, int NS, int DT, int NT>
+__device__ __forceinline__ void pass_mfma16(float *re, float *im, const float2 *__restrict__ tw, const MfW<DT, NT> &W,
+                                            int LP, int nf)
+{
+    constexpr int NBF = L / 16;  // butterflies per transform (== T)
+    static_assert(NBF == T, "MFMA stage expects T = L / 16");
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int j = lane & 15, gq = lane >> 4;
+    float xr[4][4], xi[4][4];
+    int fls[4], bs[4];
+    // gather: block u of this wave covers global butterflies (4 wave + u) * 16 + 0..15
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int g = (wave * 4 + u) * 16 + j;
+        const int fl = g / NBF;
+        const int b = g - fl * NBF;
+        fls[u] = fl;
+        bs[u] = b;
+        const bool act = fl < nf;
+        const int lbase = (act ? fl : 0) * LP;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int r = 4 * gq + t;
+            float vr = 0.f, vi = 0.f;
+            if (act) {
+                const int jl = lds_idx_read<NBF>(b, r);
+                vr = re[lbase + jl];
+                vi = im[lbase + jl];
+                if constexpr (NS > 1) {
+                    if (r > 0) {
+                        const float2 w = tw[(b % NS) * 15 + r - 1];
+                        const float2 p = cmul(make_float2(vr, vi), w);
+                        vr = p.x;
+                        vi = p.y;
+                    }
+                }
+            }
+            xr[u][t] = vr;
+            xi[u][t] = vi;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        float scale = 1.f, unscale = 1.f;
+        if constexpr (DT == MF_F16) {
+            // per-column power-of-two scale: the column lives in lanes j, j+16, j+32, j+48
+            float m = 0.f;
+#pragma unroll
+            for (int t = 0; t < 4; t++) m = fmaxf(m, fmaxf(fabsf(xr[u][t]), fabsf(xi[u][t])));
+            m = fmaxf(m, __shfl_xor(m, 16));
+            m = fmaxf(m, __shfl_xor(m, 32));
+            if (m > 0.f) {
+                int e = (int)((__float_as_uint(m) >> 23) & 255u) - 127;  // m in [2^e, 2^(e+1))
+                e = e < -100 ? -100 : (e > 100 ? 100 : e);
+                scale = __uint_as_float((unsigned)(127 + 13 - e) << 23);
+                unscale = __uint_as_float((unsigned)(127 - 13 + e) << 23);
+            }
+        }
+        typename MfOp<DT>::vec br[NT], bi[NT];
+        mf_split<DT, NT>(xr[u], scale, br);
+        mf_split<DT, NT>(xi[u], scale, bi);
+        mf_float4 acc[2];
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++) {
+            acc[rb] = mf_float4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kb = 0; kb < 2; kb++) {
+                const auto *bb = kb == 0 ? br : bi;
+                // hi*hi, hi*lo(.), lo(.)*hi and, for three terms, hi*lo, lo*hi, mid*mid
+                acc[rb] = MfOp<DT>::mma(W.a[0][rb][kb], bb[0], acc[rb]);
+                acc[rb] = MfOp<DT>::mma(W.a[0][rb][kb], bb[1], acc[rb]);
+                acc[rb] = MfOp<DT>::mma(W.a[1][rb][kb], bb[0], acc[rb]);
+                if constexpr (NT == 3) {
+                    acc[rb] = MfOp<DT>::mma(W.a[0][rb][kb], bb[2], acc[rb]);
+                    acc[rb] = MfOp<DT>::mma(W.a[2][rb][kb], bb[0], acc[rb]);
+                    acc[rb] = MfOp<DT>::mma(W.a[1][rb][kb], bb[1], acc[rb]);
+                }
+            }
+        }
+        const int fl = fls[u];
+        if (fl < nf) {
+            const int b = bs[u];
+            const int k = b % NS;
+            const int hi = b / NS;
+            const int lbase = fl * LP;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int q = 4 * gq + t;
+                const int jl = lds_idx_write<NS, 16>(hi, k, q);
+                re[lbase + jl] = acc[0][t] * unscale;
+                im[lbase + jl] = acc[1][t] * unscale;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Radix chain with radix-16 passes on MFMA and the others on the VALU (pass_static).
+template <int L, int T, int S, int DT, int NT, int NS, int R, int... Rest>
+__device__ __forceinline__ void passes_mixed(float *re, float *im, const float2 *__restrict__ tw,
+                                             const MfW<DT, NT> &W, int LP, int nf, int lbase, int t, bool active)
+{
+    if constexpr (R == 16)
+        pass_mfma16<L, T, S, NS, DT, NT>(re, im, tw, W, LP, nf);
+    else
+        pass_static<L, T, S, NS, R, false, false>(re, im, tw, lbase, t, active, nullptr, nullptr);
+    if constexpr (sizeof...(Rest) > 0)
+        passes_mixed<L, T, S, DT, NT, NS * R, Rest...>(re, im, tw + NS * (R - 1), W, LP, nf, lbase, t, active);
+}
+
+template <int L, int T, int S, int DT, int NT, int... Rs>
+struct MfmaRunner {
+    __device__ __forceinline__ void operator()(const StockhamArgs &a, float *re, float *im, int lbase, int t,
+                                               bool active, const float2 *, float2 *, int nf) const
+    {
+        MfW<DT, NT> W;
+        W.load(a.mfma_w, (int)(threadIdx.x & 63));
+        passes_mixed<L, T, S, DT, NT, 1, Rs...>(re, im, a.tw, W, a.LP, nf, lbase, t, active);
+    }
+};
+
+// Tile kernel with the radix-16 stages on the matrix cores (LDS-staged load and store).
+template <int L, int T, int S, int DT, int NT, int... Rs>
+__global__ __launch_bounds__(1024) void stockham_mfma_kernel(const StockhamArgs a)
+{
+    extern __shared__ float lds[];
+    stockham_body<L, T, S>(a, lds, MfmaRunner<L, T, S, DT, NT, Rs...>{});
+}
+
+}  // namespace sip

@@ -52,6 +52,7 @@ struct StockhamArgs {
     const float2 *rtw;     // real-transform roots w^k = e^{s 2 pi i k/(2L)}, k = 0..L
     const float2 *tw4_lo;  // four-step twiddles W^m = lo[m & mask] * hi[m >> shift], W = e^{s 2 pi i/Ntot}
     const float2 *tw4_hi;
+    const unsigned short *mfma_w;  // split DFT-matrix fragments of the MFMA radix-16 stages
     int64_t nfft;          // number of transforms in this launch (< 2^31)
     int64_t tw4_ntot;      // 0 = no four-step twiddle
     int64_t tw4_mask;
@@ -74,6 +75,14 @@ struct StockhamArgs {
     FastDiv div_in_vec;    // divides by in_len / 2 (16-byte path)
     FastDiv div_out_vec;   // divides by out_len / 2
     TileDesc tin, tout;
+};
+
+// Precision modes of the radix-16 stages (plain = FP32 VALU butterflies).
+enum PrecisionMode : int32_t {
+    PREC_FP32 = 0,       // plain form
+    PREC_F16_SPLIT = 1,  // MFMA, FP16 hi+lo operands, per-column power-of-two scaling
+    PREC_BF16_SPLIT = 2, // MFMA, BF16 hi+lo operands
+    PREC_BF16_SPLIT3 = 3 // MFMA, BF16 hi+mid+lo operands
 };
 
 // Host-side view of one compiled kernel instance.

@@ -318,10 +318,28 @@ __global__ __launch_bounds__(1024) void stockham_rows_kernel(const StockhamArgs 
     split_rows<L, T, S, Rs...>(a, lds, tile_index(a));
 }
 
+// Pass runners: the radix chain the tile goes through between load and store.
 template <int L, int T, int S, int... Rs>
-__global__ __launch_bounds__(1024) void stockham_kernel(const StockhamArgs a)
+struct StaticRunner {
+    __device__ __forceinline__ void operator()(const StockhamArgs &a, float *re, float *im, int lbase, int t,
+                                               bool active, const float2 *gin, float2 *gout, int nf) const
+    {
+        if (a.direct_in && a.direct_out)
+            passes_static<L, T, S, true, true, 1, Rs...>(re, im, a.tw, lbase, t, active, gin, gout);
+        else if (a.direct_in)
+            passes_static<L, T, S, true, false, 1, Rs...>(re, im, a.tw, lbase, t, active, gin, gout);
+        else if (a.direct_out)
+            passes_static<L, T, S, false, true, 1, Rs...>(re, im, a.tw, lbase, t, active, gin, gout);
+        else
+            passes_static<L, T, S, false, false, 1, Rs...>(re, im, a.tw, lbase, t, active, gin, gout);
+    }
+};
+
+// Generic tile kernel body: per-transform bases, tile load (with C2R pre-processing), the runner's
+// radix passes, tile store (with R2C post-processing / four-step twiddle).
+template <int L, int T, int S, class Runner>
+__device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds, const Runner &run)
 {
-    extern __shared__ float lds[];
     const int nthreads = blockDim.x;
     const int tid = threadIdx.x;
     const int F = a.F;
@@ -481,14 +499,7 @@ __global__ __launch_bounds__(1024) void stockham_kernel(const StockhamArgs a)
         const int fsel = active ? flp : 0;
         const float2 *gin = static_cast<const float2 *>(a.in) + (a.direct_in ? in_base[fsel] : 0);
         float2 *gout = static_cast<float2 *>(a.out) + (a.direct_out ? out_base[fsel] : 0);
-        if (a.direct_in && a.direct_out)
-            passes_static<L, T, S, true, true, 1, Rs...>(re, im, a.tw, lbase, t, active, gin, gout);
-        else if (a.direct_in)
-            passes_static<L, T, S, true, false, 1, Rs...>(re, im, a.tw, lbase, t, active, gin, gout);
-        else if (a.direct_out)
-            passes_static<L, T, S, false, true, 1, Rs...>(re, im, a.tw, lbase, t, active, gin, gout);
-        else
-            passes_static<L, T, S, false, false, 1, Rs...>(re, im, a.tw, lbase, t, active, gin, gout);
+        run(a, re, im, lbase, t, active, gin, gout, nf);
     }
     if (a.direct_out) return;
 
@@ -569,6 +580,13 @@ __global__ __launch_bounds__(1024) void stockham_kernel(const StockhamArgs a)
             }
         }
     }
+}
+
+template <int L, int T, int S, int... Rs>
+__global__ __launch_bounds__(1024) void stockham_kernel(const StockhamArgs a)
+{
+    extern __shared__ float lds[];
+    stockham_body<L, T, S>(a, lds, StaticRunner<L, T, S, Rs...>{});
 }
 
 }  // namespace sip

@@ -24,8 +24,10 @@ def to_dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
-def make(dims, fft_type, direction, batch, dim_type=S.ASCEND_FFT_HORIZONTAL):
+def make(dims, fft_type, direction, batch, dim_type=S.ASCEND_FFT_HORIZONTAL, precision=0):
     h = S.FftHandle()
+    if precision:
+        assert h.set_precision(precision) == S.ACL_SUCCESS
     if len(dims) == 1:
         st = h.make_plan_1d(dims[0], fft_type, direction, batch, dim_type)
     elif len(dims) == 2:
@@ -51,10 +53,10 @@ def run(h, kind, x_host, out_shape, out_dtype):
     return yd.cpu().numpy()
 
 
-def c2c(x, dims, direction=FWD, dim_type=S.ASCEND_FFT_HORIZONTAL):
+def c2c(x, dims, direction=FWD, dim_type=S.ASCEND_FFT_HORIZONTAL, precision=0):
     torch = _torch()
     batch = x.size // int(np.prod(dims))
-    h = make(dims, S.ASCEND_FFT_C2C, direction, batch, dim_type)
+    h = make(dims, S.ASCEND_FFT_C2C, direction, batch, dim_type, precision)
     try:
         return run(h, "c2c", x, x.shape, torch.complex64)
     finally:

@@ -211,3 +211,14 @@ int main() {
     subprocess.run(cmd, check=True, capture_output=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and "ok" in r.stdout, (r.returncode, r.stdout, r.stderr)
+
+
+def test_set_precision_validation():
+    """sipFftSetPrecision accepts modes 0..3 only and rejects unknown handles with 100000."""
+    h = S.FftHandle()
+    try:
+        assert h.set_precision(7) == S.ACL_ERROR_INVALID_PARAM
+        assert h.set_precision(-1) == S.ACL_ERROR_INVALID_PARAM
+    finally:
+        h.destroy()
+    assert S.lib().sipFftSetPrecision(None, 1) == S.ACL_ERROR_INVALID_PARAM
