@@ -108,8 +108,11 @@ int32_t sipFftBatchDestroy(void *handle);
  * input is a device address; output a device address, or NULL to let the server allocate it from the
  * arena (the result is then discarded after completion). Wait blocks until the request finished and
  * returns its status and latency (arrival to observed completion). Stats returns JSON
- * (p50/p90/p99 latency, launches, coalescing, arena use). Benchmark runs an open-loop Poisson
- * stream (see serving.cpp). Calls that overlap on one server are serialised internally. */
+ * (p50/p90/p99 latency, launches, coalescing, arena use); with reset = 1 the counters are cleared
+ * only when the report fits `cap` (a size query with buf = NULL clears nothing). Benchmark runs an
+ * open-loop Poisson stream (see serving.cpp) once per call and writes its JSON report into buf: pass
+ * a buffer of 64 KiB (a short buffer loses the report and returns 100000 with *needed set). Calls
+ * that overlap on one server are serialised internally. */
 int32_t sipFftServeCreate(void **handle, size_t budget_bytes, int32_t nstreams, int64_t max_coalesce_rows);
 int32_t sipFftServeSubmit(void *handle, int32_t fftType, int64_t n, int64_t batch, const void *input, void *output,
                           uint64_t *ticket);

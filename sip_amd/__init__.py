@@ -288,6 +288,15 @@ def _describe(call) -> str:
     return buf.value.decode()
 
 
+def _run_report(call, cap=1 << 20) -> str:
+    """One call into an entry point that runs work and reports JSON (it must not run twice)."""
+    need = ctypes.c_size_t()
+    buf = ctypes.create_string_buffer(cap)
+    st = call(buf, cap, ctypes.byref(need))
+    _check(st, "report")
+    return buf.value.decode()
+
+
 def describe_dry(dims, fft_type, direction, batch, dim_type=ASCEND_FFT_HORIZONTAL) -> dict:
     """Host-only plan layout (no GPU): stages, kernels, workspace, and the reference's step list."""
     dims = [int(d) for d in dims]
@@ -410,11 +419,11 @@ class Server:
         return lib().sipFftServeDrain(self.h)
 
     def stats(self, reset=False) -> dict:
-        return json.loads(_describe(lambda buf, cap, need: lib().sipFftServeStats(self.h, int(reset), buf, cap, need)))
+        return json.loads(_run_report(lambda buf, cap, need: lib().sipFftServeStats(self.h, int(reset), buf, cap, need)))
 
     def benchmark(self, rate, seconds, pool_ptr, pool_bytes, seed=1, c2c_frac=0.7, n_lo=256, n_hi=32768, b_max=512,
                   drain_seconds=30.0) -> dict:
-        return json.loads(_describe(lambda buf, cap, need: lib().sipFftServeBenchmark(
+        return json.loads(_run_report(lambda buf, cap, need: lib().sipFftServeBenchmark(
             self.h, rate, seconds, seed, ctypes.c_void_p(int(pool_ptr)), int(pool_bytes), c2c_frac, n_lo, n_hi,
             b_max, drain_seconds, buf, cap, need)))
 

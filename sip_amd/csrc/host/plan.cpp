@@ -63,8 +63,9 @@ static TableRef pass_twiddles(const StockhamInstance *in, int sign)
         for (int p = 0; p < in->npass; p++) {
             const int64_t R = in->radix[p];
             const int64_t step = L / (ns * R);
-            for (int64_t k = 0; k < ns; k++)
-                for (int64_t r = 1; r < R; r++) v.push_back(root_f(k * r * step, L, sign));
+            // [r - 1][k]: lanes of a wave (consecutive k) read consecutive entries
+            for (int64_t r = 1; r < R; r++)
+                for (int64_t k = 0; k < ns; k++) v.push_back(root_f(k * r * step, L, sign));
             ns *= R;
         }
         if (v.empty()) v.push_back(root_f(0, 1, 1));
@@ -422,7 +423,6 @@ struct Builder {
         // Measured on MI355X (N=4096 x 16384): the two-plane kernel in launch order beats the
         // single-plane exchange kernel (twice the resident workgroups) and XCD-contiguous tile order;
         // both stay available as tuning switches SIP_FFT_SPLIT_LDS=1 and SIP_FFT_XCD=1.
-        if (mf) a.direct_in = a.direct_out = 0;  // MFMA tiles are LDS-staged
         const bool rows = io_mode == IO_C2C && a.direct_in && a.direct_out && inst->rows_fwd != nullptr;
         a.split_lds = rows && (p.tune.split_lds >= 0 ? p.tune.split_lds == 1 : env_flag("SIP_FFT_SPLIT_LDS", false));
         a.xcd_remap = p.tune.xcd >= 0 ? p.tune.xcd == 1 : env_flag("SIP_FFT_XCD", false);

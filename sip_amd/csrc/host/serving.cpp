@@ -12,8 +12,8 @@
 //     identical alone and in the mix;
 //   * spreads launches over several streams (default priority), so work admitted while other work
 //     runs overlaps on the device.
-// Plans are rebuilt per launch on the host (microseconds); twiddle tables come from the
-// device-resident coefficient cache and stay resident while the server holds them.
+// Plans are rebuilt per launch on the host (microseconds); the coefficient tables of every shape
+// served stay resident on the device for the server's lifetime (keep_resident).
 // Extension API sipFftServe*; no counterpart in the reference.
 #include <algorithm>
 #include <atomic>
@@ -210,8 +210,9 @@ public:
         done_cv_.wait(lk, [&] { return queue_.empty() && inflight_.empty(); });
     }
 
-    // latencies (ms) of requests completed since the last reset, and counters
-    std::string stats(bool reset)
+    // latencies (ms) of requests completed since the last reset, and counters; the reset happens
+    // only when the report fits `cap` bytes (a size query does not lose the data)
+    std::string stats(bool reset, size_t cap = SIZE_MAX)
     {
         std::lock_guard<std::mutex> lk(mu_);
         std::vector<double> lat = lat_ms_;
@@ -224,7 +225,7 @@ public:
            << ",\"inflight_launches\":" << inflight_.size() << ",\"arena_used\":" << arena_->used()
            << ",\"arena_peak\":" << arena_peak_ << ",\"budget\":" << budget_ << ",\"bytes_done\":" << bytes_done_
            << ",\"flops_done\":" << flops_done_ << "}";
-        if (reset) {
+        if (reset && os.str().size() + 1 <= cap) {
             lat_ms_.clear();
             launches_ = coalesced_ = 0;
             bytes_done_ = flops_done_ = 0;
@@ -441,6 +442,7 @@ private:
             done_cv_.notify_all();
             return true;
         }
+        keep_resident(type, n, *plan);
         if (plan->ws_bytes) {
             l->ws_bytes = plan->ws_bytes;
             l->ws_off = arena_->alloc(l->ws_bytes);
@@ -469,8 +471,22 @@ private:
         return true;
     }
 
+    // The coefficient tables of every (type, n) served so far stay on the device for the server's
+    // lifetime: a later launch of the same shape reuses them without a rebuild or an upload, and no
+    // table is freed (a device-synchronising hipFree) while launches run.
+    void keep_resident(int type, int64_t n, const Plan &plan)
+    {
+        auto &held = resident_[std::make_pair(type, n)];
+        if (!held.empty()) return;
+        for (const Stage &st : plan.stages)
+            for (const TableRef *t : {&st.tw, &st.rtw, &st.tw4lo, &st.tw4hi, &st.mfw, &st.tab0})
+                if (t->dev) held.push_back(t->dev);
+        if (held.empty()) held.push_back(nullptr);  // marks the shape as seen
+    }
+
     size_t budget_;
     int nstreams_;
+    std::map<std::pair<int, int64_t>, std::vector<TablePtr>> resident_;
     int64_t max_rows_;
     int device_ = 0;
     void *arena_base_ = nullptr;
@@ -587,7 +603,7 @@ int32_t sipFftServeStats(void *handle, int32_t reset, char *buf, size_t cap, siz
     return sip::guarded([&]() -> int32_t {
         sip::Server *s = sip::lookup_server(handle);
         if (!s) return E::ACL_ERROR_INVALID_PARAM;
-        const std::string j = s->stats(reset != 0);
+        const std::string j = s->stats(reset != 0, buf ? cap : 0);
         if (needed) *needed = j.size() + 1;
         if (!buf || cap < j.size() + 1) return E::ACL_ERROR_INVALID_PARAM;
         std::memcpy(buf, j.c_str(), j.size() + 1);

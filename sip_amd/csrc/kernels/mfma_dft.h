@@ -117,10 +117,12 @@ __device__ __forceinline__ void mf_split(const float (&x)[4], float scale, typen
 }
 
 // One radix-16 pass on MFMA for a tile of F transforms of length L in LDS (T = L/16 threads per
-// transform, F * T a multiple of 64). NS = product of the earlier radices.
-template <int L, int T, int S, int NS, int DT, int NT>
+// transform, F * T a multiple of 64). NS = product of the earlier radices. GIN: the pass reads its
+// inputs from HBM (src + in_base[f]); GOUT: it writes its outputs to HBM (dst + out_base[f]).
+template <int L, int T, int S, int NS, int DT, int NT, bool GIN, bool GOUT>
 __device__ __forceinline__ void pass_mfma16(float *re, float *im, const float2 *__restrict__ tw, const MfW<DT, NT> &W,
-                                            int LP, int nf)
+                                            int LP, int nf, const float2 *__restrict__ src, float2 *__restrict__ dst,
+                                            const int64_t *in_base, const int64_t *out_base)
 {
     constexpr int NBF = L / 16;  // butterflies per transform (== T)
     static_assert(NBF == T, "MFMA stage expects T = L / 16");
@@ -144,12 +146,18 @@ __device__ __forceinline__ void pass_mfma16(float *re, float *im, const float2 *
             const int r = 4 * gq + t;
             float vr = 0.f, vi = 0.f;
             if (act) {
-                const int jl = lds_idx_read<NBF>(b, r);
-                vr = re[lbase + jl];
-                vi = im[lbase + jl];
+                if constexpr (GIN) {
+                    const float2 v = gload(src + in_base[fl] + b + r * NBF);
+                    vr = v.x;
+                    vi = v.y;
+                } else {
+                    const int jl = lds_idx_read<NBF>(b, r);
+                    vr = re[lbase + jl];
+                    vi = im[lbase + jl];
+                }
                 if constexpr (NS > 1) {
                     if (r > 0) {
-                        const float2 w = tw[(b % NS) * 15 + r - 1];
+                        const float2 w = tw[(r - 1) * NS + b % NS];
                         const float2 p = cmul(make_float2(vr, vi), w);
                         vr = p.x;
                         vi = p.y;
@@ -160,7 +168,7 @@ __device__ __forceinline__ void pass_mfma16(float *re, float *im, const float2 *
             xi[u][t] = vi;
         }
     }
-    __syncthreads();
+    if constexpr (!GIN) __syncthreads();
 #pragma unroll
     for (int u = 0; u < 4; u++) {
         float scale = 1.f, unscale = 1.f;
@@ -208,40 +216,68 @@ __device__ __forceinline__ void pass_mfma16(float *re, float *im, const float2 *
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 const int q = 4 * gq + t;
-                const int jl = lds_idx_write<NS, 16>(hi, k, q);
-                re[lbase + jl] = acc[0][t] * unscale;
-                im[lbase + jl] = acc[1][t] * unscale;
+                if constexpr (GOUT) {
+                    gstore(dst + out_base[fl] + hi * NS * 16 + k + q * NS,
+                           make_float2(acc[0][t] * unscale, acc[1][t] * unscale));
+                } else {
+                    const int jl = lds_idx_write<NS, 16>(hi, k, q);
+                    re[lbase + jl] = acc[0][t] * unscale;
+                    im[lbase + jl] = acc[1][t] * unscale;
+                }
             }
         }
     }
-    __syncthreads();
+    if constexpr (!GOUT) __syncthreads();
 }
 
-// Radix chain with radix-16 passes on MFMA and the others on the VALU (pass_static).
-template <int L, int T, int S, int DT, int NT, int NS, int R, int... Rest>
+// Radix chain with radix-16 passes on MFMA and the others on the VALU (pass_static). DIN / DOUT:
+// the first pass reads HBM / the last pass writes HBM directly.
+struct MixedIo {
+    const float2 *src;           // tile input (MFMA passes, per-block transform bases)
+    float2 *dst;
+    const int64_t *in_base, *out_base;
+    const float2 *gin;           // this thread's transform (VALU passes)
+    float2 *gout;
+};
+
+template <int L, int T, int S, int DT, int NT, bool DIN, bool DOUT, int NS, int R, int... Rest>
 __device__ __forceinline__ void passes_mixed(float *re, float *im, const float2 *__restrict__ tw,
-                                             const MfW<DT, NT> &W, int LP, int nf, int lbase, int t, bool active)
+                                             const MfW<DT, NT> &W, int LP, int nf, int lbase, int t, bool active,
+                                             const MixedIo &io)
 {
+    constexpr bool first = NS == 1, last = sizeof...(Rest) == 0;
     if constexpr (R == 16)
-        pass_mfma16<L, T, S, NS, DT, NT>(re, im, tw, W, LP, nf);
+        pass_mfma16<L, T, S, NS, DT, NT, DIN && first, DOUT && last>(re, im, tw, W, LP, nf, io.src, io.dst,
+                                                                     io.in_base, io.out_base);
     else
-        pass_static<L, T, S, NS, R, false, false>(re, im, tw, lbase, t, active, nullptr, nullptr);
-    if constexpr (sizeof...(Rest) > 0)
-        passes_mixed<L, T, S, DT, NT, NS * R, Rest...>(re, im, tw + NS * (R - 1), W, LP, nf, lbase, t, active);
+        pass_static<L, T, S, NS, R, DIN && first, DOUT && last>(re, im, tw, lbase, t, active, io.gin, io.gout);
+    if constexpr (!last)
+        passes_mixed<L, T, S, DT, NT, DIN, DOUT, NS * R, Rest...>(re, im, tw + NS * (R - 1), W, LP, nf, lbase, t,
+                                                                 active, io);
 }
 
 template <int L, int T, int S, int DT, int NT, int... Rs>
 struct MfmaRunner {
     __device__ __forceinline__ void operator()(const StockhamArgs &a, float *re, float *im, int lbase, int t,
-                                               bool active, const float2 *, float2 *, int nf) const
+                                               bool active, const float2 *gin, float2 *gout, int nf,
+                                               const int64_t *in_base, const int64_t *out_base) const
     {
         MfW<DT, NT> W;
         W.load(a.mfma_w, (int)(threadIdx.x & 63));
-        passes_mixed<L, T, S, DT, NT, 1, Rs...>(re, im, a.tw, W, a.LP, nf, lbase, t, active);
+        const MixedIo io{static_cast<const float2 *>(a.in), static_cast<float2 *>(a.out), in_base, out_base,
+                         gin, gout};
+        if (a.direct_in && a.direct_out)
+            passes_mixed<L, T, S, DT, NT, true, true, 1, Rs...>(re, im, a.tw, W, a.LP, nf, lbase, t, active, io);
+        else if (a.direct_in)
+            passes_mixed<L, T, S, DT, NT, true, false, 1, Rs...>(re, im, a.tw, W, a.LP, nf, lbase, t, active, io);
+        else if (a.direct_out)
+            passes_mixed<L, T, S, DT, NT, false, true, 1, Rs...>(re, im, a.tw, W, a.LP, nf, lbase, t, active, io);
+        else
+            passes_mixed<L, T, S, DT, NT, false, false, 1, Rs...>(re, im, a.tw, W, a.LP, nf, lbase, t, active, io);
     }
 };
 
-// Tile kernel with the radix-16 stages on the matrix cores (LDS-staged load and store).
+// Tile kernel with the radix-16 stages on the matrix cores.
 template <int L, int T, int S, int DT, int NT, int... Rs>
 __global__ __launch_bounds__(1024) void stockham_mfma_kernel(const StockhamArgs a)
 {

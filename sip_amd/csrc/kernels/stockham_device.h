@@ -117,9 +117,8 @@ __device__ __forceinline__ void pass_static(float *re, float *im, const float2 *
             }
             if constexpr (NS > 1) {
                 const int k = b % NS;
-                const float2 *tp = tw + k * (R - 1);
 #pragma unroll
-                for (int r = 1; r < R; r++) w[q][r] = tp[r - 1];
+                for (int r = 1; r < R; r++) w[q][r] = tw[(r - 1) * NS + k];
             }
         }
     }
@@ -209,9 +208,8 @@ __device__ __forceinline__ void split_chain(float *pl, const float2 *__restrict_
         if (active && (EXACT || b < NBF)) {
             if constexpr (NS > 1) {
                 const int k = b % NS;
-                const float2 *tp = tw + k * (R - 1);
 #pragma unroll
-                for (int r = 1; r < R; r++) v[q][r] = cmul(v[q][r], tp[r - 1]);
+                for (int r = 1; r < R; r++) v[q][r] = cmul(v[q][r], tw[(r - 1) * NS + k]);
             }
             dft<R, S>(v[q]);
         }
@@ -322,7 +320,8 @@ __global__ __launch_bounds__(1024) void stockham_rows_kernel(const StockhamArgs 
 template <int L, int T, int S, int... Rs>
 struct StaticRunner {
     __device__ __forceinline__ void operator()(const StockhamArgs &a, float *re, float *im, int lbase, int t,
-                                               bool active, const float2 *gin, float2 *gout, int nf) const
+                                               bool active, const float2 *gin, float2 *gout, int nf,
+                                               const int64_t *, const int64_t *) const
     {
         if (a.direct_in && a.direct_out)
             passes_static<L, T, S, true, true, 1, Rs...>(re, im, a.tw, lbase, t, active, gin, gout);
@@ -499,7 +498,7 @@ __device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds,
         const int fsel = active ? flp : 0;
         const float2 *gin = static_cast<const float2 *>(a.in) + (a.direct_in ? in_base[fsel] : 0);
         float2 *gout = static_cast<float2 *>(a.out) + (a.direct_out ? out_base[fsel] : 0);
-        run(a, re, im, lbase, t, active, gin, gout, nf);
+        run(a, re, im, lbase, t, active, gin, gout, nf, in_base, out_base);
     }
     if (a.direct_out) return;
 
