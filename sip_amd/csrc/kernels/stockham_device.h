@@ -386,33 +386,54 @@ __device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds,
             }
         }
     } else if (a.tin.vec2) {
+        // the block is F * T threads: every thread moves at most NV float4 (all loads in flight
+        // before the first LDS store); in_len is at most L + 1 (C2R half spectrum)
+        constexpr int NV = ((L + 1) / 2 + T - 1) / T;
         const float4 *src = static_cast<const float4 *>(a.in);
         const int half = in_len >> 1;
         const int total = F * half;
-        for (int i = tid; i < total; i += nthreads) {
+        float4 v[NV];
+#pragma unroll
+        for (int u = 0; u < NV; u++) {
+            const int i = tid + u * nthreads;
             const int fl = (int)fdiv((uint32_t)i, a.div_in_vec);
-            const int jj = i - fl * half;
-            if (fl < nf) {
-                const float4 v = src[(in_base[fl] >> 1) + jj];
-                const int j = 2 * jj;
-                re[fl * LP + lds_pad(j)] = v.x;
-                im[fl * LP + lds_pad(j)] = v.y;
-                re[fl * LP + lds_pad(j + 1)] = v.z;
-                im[fl * LP + lds_pad(j + 1)] = v.w;
+            if (i < total && fl < nf) v[u] = src[(in_base[fl] >> 1) + (i - fl * half)];
+        }
+#pragma unroll
+        for (int u = 0; u < NV; u++) {
+            const int i = tid + u * nthreads;
+            const int fl = (int)fdiv((uint32_t)i, a.div_in_vec);
+            if (i < total && fl < nf) {
+                const int j = 2 * (i - fl * half);
+                re[fl * LP + lds_pad(j)] = v[u].x;
+                im[fl * LP + lds_pad(j)] = v[u].y;
+                re[fl * LP + lds_pad(j + 1)] = v[u].z;
+                im[fl * LP + lds_pad(j + 1)] = v[u].w;
             }
         }
     } else if (a.tin.unit_f && a.col_fixed) {
-        // column tile: each thread keeps one transform (fl = tid mod F) and walks its elements
-        const int fl = tid - (int)fdiv((uint32_t)tid, a.div_F) * F;
-        const int jstep = nthreads / F;
+        // column tile: each thread keeps one transform (fl = tid mod F) and loads its elements
+        // j0, j0 + T, ... (the block is F * T threads), all loads in flight before the LDS stores
+        constexpr int NU = (L + T - 1) / T;
+        const int j0 = (int)fdiv((uint32_t)tid, a.div_F);
+        const int fl = tid - j0 * F;
         if (fl < nf) {
             const float2 *p = static_cast<const float2 *>(a.in) + in_base[fl];
             const int64_t es = a.tin.es;
             float *pr = re + fl * LP, *pi = im + fl * LP;
-            for (int j = (int)fdiv((uint32_t)tid, a.div_F); j < in_len; j += jstep) {
-                const float2 v = p[(int64_t)j * es];
-                pr[lds_pad(j)] = v.x;
-                pi[lds_pad(j)] = v.y;
+            float2 v[NU];
+#pragma unroll
+            for (int u = 0; u < NU; u++) {
+                const int j = j0 + u * T;
+                if (j < in_len) v[u] = p[(int64_t)j * es];
+            }
+#pragma unroll
+            for (int u = 0; u < NU; u++) {
+                const int j = j0 + u * T;
+                if (j < in_len) {
+                    pr[lds_pad(j)] = v[u].x;
+                    pi[lds_pad(j)] = v[u].y;
+                }
             }
         }
     } else {
@@ -535,12 +556,16 @@ __device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds,
             if (fl < nf) dst[out_base[fl] + (int64_t)j * a.tout.es] = re[fl * LP + lds_pad(j)];
         }
     } else if (a.tout.vec2 && a.tw4_ntot == 0) {
+        constexpr int NV = ((L + 1) / 2 + T - 1) / T;
         float4 *dst = static_cast<float4 *>(a.out);
         const int half = out_len >> 1;
-        for (int i = tid; i < F * half; i += nthreads) {
+        const int total = F * half;
+#pragma unroll
+        for (int u = 0; u < NV; u++) {
+            const int i = tid + u * nthreads;
             const int fl = (int)fdiv((uint32_t)i, a.div_out_vec);
-            const int jj = i - fl * half;
-            if (fl < nf) {
+            if (i < total && fl < nf) {
+                const int jj = i - fl * half;
                 const int j = 2 * jj;
                 const float *pr = re + fl * LP, *pi = im + fl * LP;
                 dst[(out_base[fl] >> 1) + jj] =
@@ -548,17 +573,35 @@ __device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds,
             }
         }
     } else if (a.tout.unit_f && a.col_fixed) {
-        const int fl = tid - (int)fdiv((uint32_t)tid, a.div_F) * F;
-        const int jstep = nthreads / F;
+        constexpr int NU = (L + T - 1) / T;
+        const int j0 = (int)fdiv((uint32_t)tid, a.div_F);
+        const int fl = tid - j0 * F;
         if (fl < nf) {
             float2 *p = static_cast<float2 *>(a.out) + out_base[fl];
             const int64_t es = a.tout.es;
             const float *pr = re + fl * LP, *pi = im + fl * LP;
-            const int64_t c = col[fl];
-            for (int j = (int)fdiv((uint32_t)tid, a.div_F); j < out_len; j += jstep) {
-                float2 v = make_float2(pr[lds_pad(j)], pi[lds_pad(j)]);
-                if (a.tw4_ntot != 0) v = cmul(v, tw4_root(a, c * (int64_t)j));
-                p[(int64_t)j * es] = v;
+            float2 v[NU];
+#pragma unroll
+            for (int u = 0; u < NU; u++) {
+                const int j = j0 + u * T;
+                if (j < out_len) v[u] = make_float2(pr[lds_pad(j)], pi[lds_pad(j)]);
+            }
+            if (a.tw4_ntot != 0) {
+                // four-step twiddle: all table loads issued before the first multiply
+                const int64_t c = col[fl];
+                float2 w[NU];
+#pragma unroll
+                for (int u = 0; u < NU; u++) {
+                    const int j = j0 + u * T;
+                    if (j < out_len) w[u] = tw4_root(a, c * (int64_t)j);
+                }
+#pragma unroll
+                for (int u = 0; u < NU; u++) v[u] = cmul(v[u], w[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < NU; u++) {
+                const int j = j0 + u * T;
+                if (j < out_len) p[(int64_t)j * es] = v[u];
             }
         }
     } else {
