@@ -382,9 +382,17 @@ struct Builder {
         const size_t per = (size_t)2 * lp * 4 + 24;
         int F;
         if (column) {
-            // aim at >= 256-byte runs per row of the tile (32 transforms), within 80 KiB of LDS
-            F = p.tune.col_F > 0 ? p.tune.col_F : std::min(64, std::max(32, 1024 / T));
-            while (F > 1 && ((size_t)F * per > 80 * 1024 || F * T > 1024)) F /= 2;
+            // Tile width: a multiple of 16 transforms (each row run covers whole 128-byte lines),
+            // about 512 threads per workgroup, at least 16 wide even when that takes the whole
+            // LDS (1024 x 16: 947 us against 1126 us at width 8 per 1 GiB 2D pass on MI355X),
+            // at most 64.
+            int want = p.tune.col_F;
+            if (want <= 0) {
+                if (const char *e = std::getenv("SIP_FFT_COL_F")) want = std::atoi(e);
+            }
+            F = want > 0 ? want : std::min(64, std::max(16, (512 / T) & ~15));
+            if (F >= 16) F &= ~15;
+            while (F > 1 && ((size_t)F * per > 160 * 1024 || F * T > 1024)) F /= 2;
         } else {
             F = std::max(1, 256 / T);
             while (F > 1 && ((size_t)F * per > 48 * 1024 || F * T > 1024)) F--;
@@ -427,6 +435,10 @@ struct Builder {
         // single-plane exchange by default where two planes leave room for only one workgroup per CU
         // (16384: 479 us against 646 us per 1 GiB on MI355X)
         const bool one_per_cu = (size_t)2 * F * lp * sizeof(float) > 80 * 1024;
+        // column tiles (C2C, strided on both sides): first / last pass straight from / to HBM with
+        // lanes walking adjacent columns
+        a.col_direct = (io_mode == IO_C2C && a.tin.unit_f && a.tout.unit_f && !mf &&
+                        env_flag("SIP_FFT_COL_DIRECT", true)) ? 1 : 0;
         a.split_lds = rows && (p.tune.split_lds >= 0 ? p.tune.split_lds == 1
                                                      : env_flag("SIP_FFT_SPLIT_LDS", one_per_cu));
         a.xcd_remap = p.tune.xcd >= 0 ? p.tune.xcd == 1 : env_flag("SIP_FFT_XCD", false);
@@ -944,7 +956,7 @@ std::string plan_describe(const Plan &p)
                << ",\"grid\":" << s.grid.x << ",\"block\":" << s.block.x << ",\"unit_f_in\":" << s.sa.tin.unit_f
                << ",\"unit_f_out\":" << s.sa.tout.unit_f << ",\"tw4\":" << s.sa.tw4_ntot
                << ",\"direct_in\":" << s.sa.direct_in << ",\"direct_out\":" << s.sa.direct_out
-               << ",\"split_lds\":" << s.sa.split_lds << ",\"mfma\":" << (s.mfw.used() ? 1 : 0)
+               << ",\"split_lds\":" << s.sa.split_lds << ",\"col_direct\":" << s.sa.col_direct << ",\"mfma\":" << (s.mfw.used() ? 1 : 0)
                << ",\"xcd_remap\":" << s.sa.xcd_remap;
         } else {
             os << ",\"mode\":" << s.xa.mode << ",\"rows\":" << s.xa.rows << ",\"n\":" << s.xa.n

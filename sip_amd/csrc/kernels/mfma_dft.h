@@ -250,7 +250,7 @@ __device__ __forceinline__ void passes_mixed(float *re, float *im, const float2 
         pass_mfma16<L, T, S, NS, DT, NT, DIN && first, DOUT && last>(re, im, tw, W, LP, nf, io.src, io.dst,
                                                                      io.in_base, io.out_base);
     else
-        pass_static<L, T, S, NS, R, DIN && first, DOUT && last>(re, im, tw, lbase, t, active, io.gin, io.gout);
+        pass_static<L, T, S, NS, R, DIN && first, DOUT && last>(re, im, tw, lbase, t, active, RowIo{io.gin, io.gout});
     if constexpr (!last)
         passes_mixed<L, T, S, DT, NT, DIN, DOUT, NS * R, Rest...>(re, im, tw + NS * (R - 1), W, LP, nf, lbase, t,
                                                                  active, io);
@@ -260,7 +260,7 @@ template <int L, int T, int S, int DT, int NT, int... Rs>
 struct MfmaRunner {
     __device__ __forceinline__ void operator()(const StockhamArgs &a, float *re, float *im, int lbase, int t,
                                                bool active, const float2 *gin, float2 *gout, int nf,
-                                               const int64_t *in_base, const int64_t *out_base) const
+                                               const int64_t *in_base, const int64_t *out_base, int64_t) const
     {
         MfW<DT, NT> W;
         W.load(a.mfma_w, (int)(threadIdx.x & 63));

@@ -67,7 +67,8 @@ struct StockhamArgs {
     int32_t split_lds;     // launched as the single-plane row kernel (direct rows only)
     int32_t xcd_remap;     // give each XCD a contiguous range of tiles
     int32_t col_fixed;     // column tiles: block size is a multiple of F (one transform per thread)
-    int32_t pad2_;
+    int32_t col_direct;    // column tile (C2C, strided in and out): lanes walk transforms, the first
+                           // pass reads and the last pass writes HBM directly
     FastDiv div_in_len;    // divides by in_len
     FastDiv div_out_len;   // divides by out_len
     FastDiv div_F;         // divides by F

@@ -88,14 +88,22 @@ __device__ __forceinline__ int lds_idx_write(int hi, int k, int r)
     }
 }
 
+// HBM endpoints of the direct passes: element j of this thread's transform.
+struct RowIo {  // contiguous transform
+    const float2 *__restrict__ in;
+    float2 *__restrict__ out;
+    __device__ __forceinline__ float2 load(int j) const { return gload(in + j); }
+    __device__ __forceinline__ void store(int j, float2 v) const { gstore(out + j, v); }
+};
+
 // One radix-R pass over a transform of length L held in LDS, T threads per transform,
-// NS = product of the radices of the earlier passes, TW = this pass's twiddles [k][r-1].
-// GIN: the pass reads its inputs straight from HBM (first pass of a contiguous row: element
-// b + r*L/R, coalesced across the transform's threads) instead of LDS. GOUT: the pass writes its
-// outputs straight to HBM (last pass: the autosort output index is b + r*L/R as well).
-template <int L, int T, int S, int NS, int R, bool GIN, bool GOUT>
+// NS = product of the radices of the earlier passes, TW = this pass's twiddles [r-1][k].
+// GIN: the pass reads its inputs straight from HBM (first pass: element b + r*L/R, through
+// io.load) instead of LDS. GOUT: the pass writes its outputs straight to HBM (last pass: the
+// autosort output index is b + r*L/R as well, through io.store).
+template <int L, int T, int S, int NS, int R, bool GIN, bool GOUT, class Io>
 __device__ __forceinline__ void pass_static(float *re, float *im, const float2 *__restrict__ tw, int lbase, int t,
-                                            bool active, const float2 *__restrict__ gin, float2 *__restrict__ gout)
+                                            bool active, const Io &io)
 {
     constexpr int NBF = L / R;                 // butterflies per transform
     constexpr int NB = (NBF + T - 1) / T;      // butterflies per thread
@@ -109,7 +117,7 @@ __device__ __forceinline__ void pass_static(float *re, float *im, const float2 *
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 if constexpr (GIN) {
-                    v[q][r] = gload(gin + b + r * NBF);
+                    v[q][r] = io.load(b + r * NBF);
                 } else {
                     const int j = lds_idx_read<NBF>(b, r);
                     v[q][r] = make_float2(re[lbase + j], im[lbase + j]);
@@ -138,7 +146,7 @@ __device__ __forceinline__ void pass_static(float *re, float *im, const float2 *
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 if constexpr (GOUT) {
-                    gstore(gout + d + r * NS, v[q][r]);
+                    io.store(d + r * NS, v[q][r]);
                 } else {
                     const int j = lds_idx_write<NS, R>(hi, k, r);
                     re[lbase + j] = v[q][r].x;
@@ -150,15 +158,15 @@ __device__ __forceinline__ void pass_static(float *re, float *im, const float2 *
     if constexpr (!GOUT) __syncthreads();
 }
 
-template <int L, int T, int S, bool DIN, bool DOUT, int NS, int R, int... Rest>
+template <int L, int T, int S, bool DIN, bool DOUT, int NS, int R, int... Rest, class Io>
 __device__ __forceinline__ void passes_static(float *re, float *im, const float2 *__restrict__ tw, int lbase, int t,
-                                              bool active, const float2 *__restrict__ gin, float2 *__restrict__ gout)
+                                              bool active, const Io &io)
 {
     constexpr bool first = NS == 1;
     constexpr bool last = sizeof...(Rest) == 0;
-    pass_static<L, T, S, NS, R, DIN && first, DOUT && last>(re, im, tw, lbase, t, active, gin, gout);
+    pass_static<L, T, S, NS, R, DIN && first, DOUT && last>(re, im, tw, lbase, t, active, io);
     if constexpr (!last)
-        passes_static<L, T, S, DIN, DOUT, NS * R, Rest...>(re, im, tw + NS * (R - 1), lbase, t, active, gin, gout);
+        passes_static<L, T, S, DIN, DOUT, NS * R, Rest...>(re, im, tw + NS * (R - 1), lbase, t, active, io);
 }
 
 // f = f0 + n0 * (f1 + n1 * f2)
@@ -316,21 +324,41 @@ __global__ __launch_bounds__(1024) void stockham_rows_kernel(const StockhamArgs 
     split_rows<L, T, S, Rs...>(a, lds, tile_index(a));
 }
 
+// Strided transform of a column tile: element j at in[j * es_in] / out[j * es_out], with the
+// four-step twiddle W^{col * j} applied on store when the stage carries one.
+struct ColIo {
+    const float2 *__restrict__ in;
+    float2 *__restrict__ out;
+    int64_t es_in, es_out;
+    int64_t col;
+    const StockhamArgs *a;
+    __device__ __forceinline__ float2 load(int j) const { return gload(in + (int64_t)j * es_in); }
+    __device__ __forceinline__ void store(int j, float2 v) const
+    {
+        if (a->tw4_ntot != 0) v = cmul(v, tw4_root(*a, col * (int64_t)j));
+        gstore(out + (int64_t)j * es_out, v);
+    }
+};
+
 // Pass runners: the radix chain the tile goes through between load and store.
 template <int L, int T, int S, int... Rs>
 struct StaticRunner {
     __device__ __forceinline__ void operator()(const StockhamArgs &a, float *re, float *im, int lbase, int t,
                                                bool active, const float2 *gin, float2 *gout, int nf,
-                                               const int64_t *, const int64_t *) const
+                                               const int64_t *, const int64_t *, int64_t col) const
     {
-        if (a.direct_in && a.direct_out)
-            passes_static<L, T, S, true, true, 1, Rs...>(re, im, a.tw, lbase, t, active, gin, gout);
+        const RowIo io{gin, gout};
+        if (a.col_direct) {
+            const ColIo cio{gin, gout, a.tin.es, a.tout.es, col, &a};
+            passes_static<L, T, S, true, true, 1, Rs...>(re, im, a.tw, lbase, t, active, cio);
+        } else if (a.direct_in && a.direct_out)
+            passes_static<L, T, S, true, true, 1, Rs...>(re, im, a.tw, lbase, t, active, io);
         else if (a.direct_in)
-            passes_static<L, T, S, true, false, 1, Rs...>(re, im, a.tw, lbase, t, active, gin, gout);
+            passes_static<L, T, S, true, false, 1, Rs...>(re, im, a.tw, lbase, t, active, io);
         else if (a.direct_out)
-            passes_static<L, T, S, false, true, 1, Rs...>(re, im, a.tw, lbase, t, active, gin, gout);
+            passes_static<L, T, S, false, true, 1, Rs...>(re, im, a.tw, lbase, t, active, io);
         else
-            passes_static<L, T, S, false, false, 1, Rs...>(re, im, a.tw, lbase, t, active, gin, gout);
+            passes_static<L, T, S, false, false, 1, Rs...>(re, im, a.tw, lbase, t, active, io);
     }
 };
 
@@ -366,7 +394,7 @@ __device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds,
 
     // ---------------------------------------------------------------- load tile
     const int in_len = a.in_len;
-    if (a.direct_in) {
+    if (a.direct_in || a.col_direct) {
         // the first radix pass reads HBM directly
     } else if (a.io_mode == IO_R2C_FULL) {
         const float *src = static_cast<const float *>(a.in);
@@ -512,16 +540,24 @@ __device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds,
 
     // ---------------------------------------------------------------- radix passes
     {
-        const int flp = tid / T;
-        const int t = tid - flp * T;
+        // rows: T consecutive threads per transform; column tiles with direct HBM passes:
+        // consecutive lanes walk the F transforms (adjacent columns), t = tid / F
+        int flp, t;
+        if (a.col_direct) {
+            t = (int)fdiv((uint32_t)tid, a.div_F);
+            flp = tid - t * F;
+        } else {
+            flp = tid / T;
+            t = tid - flp * T;
+        }
         const bool active = flp < nf;
         const int lbase = (active ? flp : 0) * LP;
         const int fsel = active ? flp : 0;
-        const float2 *gin = static_cast<const float2 *>(a.in) + (a.direct_in ? in_base[fsel] : 0);
-        float2 *gout = static_cast<float2 *>(a.out) + (a.direct_out ? out_base[fsel] : 0);
-        run(a, re, im, lbase, t, active, gin, gout, nf, in_base, out_base);
+        const float2 *gin = static_cast<const float2 *>(a.in) + ((a.direct_in || a.col_direct) ? in_base[fsel] : 0);
+        float2 *gout = static_cast<float2 *>(a.out) + ((a.direct_out || a.col_direct) ? out_base[fsel] : 0);
+        run(a, re, im, lbase, t, active, gin, gout, nf, in_base, out_base, col[fsel]);
     }
-    if (a.direct_out) return;
+    if (a.direct_out || a.col_direct) return;
 
     // ---------------------------------------------------------------- store tile
     const int out_len = a.out_len;
