@@ -53,7 +53,7 @@ static float2 root_f(int64_t m, int64_t n, int sign)
 static TableRef pass_twiddles(const StockhamInstance *in, int sign)
 {
     TableRef t;
-    t.key = "tw:" + std::to_string(in->L);
+    t.key = "twp:" + std::to_string(in->L);
     for (int p = 0; p < in->npass; p++) t.key += "," + std::to_string(in->radix[p]);
     t.key += ":" + std::to_string(sign);
     t.gen = [in, sign]() {
@@ -63,8 +63,11 @@ static TableRef pass_twiddles(const StockhamInstance *in, int sign)
         for (int p = 0; p < in->npass; p++) {
             const int64_t R = in->radix[p];
             const int64_t step = L / (ns * R);
-            // [r - 1][k]: lanes of a wave (consecutive k) read consecutive entries
-            for (int64_t r = 1; r < R; r++)
+            // rows [row][k] (lanes of a wave, consecutive k, read consecutive entries): r = 1, 2, 4,
+            // .. R/2 for a power-of-two radix (the kernel multiplies the others together), every
+            // r = 1 .. R-1 otherwise
+            const bool pow2 = (R & (R - 1)) == 0;
+            for (int64_t r = 1; r < R; r = pow2 ? r * 2 : r + 1)
                 for (int64_t k = 0; k < ns; k++) v.push_back(root_f(k * r * step, L, sign));
             ns *= R;
         }

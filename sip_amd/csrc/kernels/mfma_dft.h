@@ -131,6 +131,7 @@ __device__ __forceinline__ void pass_mfma16(float *re, float *im, const float2 *
     const int j = lane & 15, gq = lane >> 4;
     float xr[4][4], xi[4][4];
     int fls[4], bs[4];
+    static_assert(tw_rows<16>() == 4, "radix-16 twiddle rows are W^1, W^2, W^4, W^8");
     // gather: block u of this wave covers global butterflies (4 wave + u) * 16 + 0..15
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -141,6 +142,22 @@ __device__ __forceinline__ void pass_mfma16(float *re, float *im, const float2 *
         bs[u] = b;
         const bool act = fl < nf;
         const int lbase = (act ? fl : 0) * LP;
+        // twiddles W^{k r} of this lane's rows r = 4 gq + t from the stored powers W^1, W^2, W^4, W^8:
+        // W^r = W^{4 gq} W^t
+        float2 lane_tw[4];
+        if constexpr (NS > 1) {
+            if (act) {
+                const int k = b % NS;
+                const float2 w1 = tw[k], w2 = tw[NS + k], w4 = tw[2 * NS + k], w8 = tw[3 * NS + k];
+                const float2 w3 = cmul(w1, w2);
+                const float2 one = make_float2(1.f, 0.f);
+                const float2 hi = gq == 0 ? one : gq == 1 ? w4 : gq == 2 ? w8 : cmul(w4, w8);
+                lane_tw[0] = hi;
+                lane_tw[1] = gq == 0 ? w1 : cmul(hi, w1);
+                lane_tw[2] = gq == 0 ? w2 : cmul(hi, w2);
+                lane_tw[3] = gq == 0 ? w3 : cmul(hi, w3);
+            }
+        }
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             const int r = 4 * gq + t;
@@ -157,8 +174,7 @@ __device__ __forceinline__ void pass_mfma16(float *re, float *im, const float2 *
                 }
                 if constexpr (NS > 1) {
                     if (r > 0) {
-                        const float2 w = tw[(r - 1) * NS + b % NS];
-                        const float2 p = cmul(make_float2(vr, vi), w);
+                        const float2 p = cmul(make_float2(vr, vi), lane_tw[t]);
                         vr = p.x;
                         vi = p.y;
                     }
@@ -252,7 +268,7 @@ __device__ __forceinline__ void passes_mixed(float *re, float *im, const float2 
     else
         pass_static<L, T, S, NS, R, DIN && first, DOUT && last>(re, im, tw, lbase, t, active, RowIo{io.gin, io.gout});
     if constexpr (!last)
-        passes_mixed<L, T, S, DT, NT, DIN, DOUT, NS * R, Rest...>(re, im, tw + NS * (R - 1), W, LP, nf, lbase, t,
+        passes_mixed<L, T, S, DT, NT, DIN, DOUT, NS * R, Rest...>(re, im, tw + NS * tw_rows<R>(), W, LP, nf, lbase, t,
                                                                  active, io);
 }
 

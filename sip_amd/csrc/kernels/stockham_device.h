@@ -88,6 +88,49 @@ __device__ __forceinline__ int lds_idx_write(int hi, int k, int r)
     }
 }
 
+// Twiddle rows of a radix-R pass. A power-of-two radix stores W^{k r} for r = 1, 2, 4, .. R/2 only
+// (the other r are products of those, at most three multiplies deep, so a pass loads log2(R)
+// twiddles instead of R - 1); other radices store every r = 1 .. R-1. Rows are [row][k], k < NS.
+template <int R>
+constexpr int tw_rows()
+{
+    if constexpr ((R & (R - 1)) == 0) {
+        int n = 0;
+        for (int p = 1; p < R; p <<= 1) n++;
+        return n;
+    } else {
+        return R - 1;
+    }
+}
+
+template <int R, int NS>
+__device__ __forceinline__ void tw_load(const float2 *__restrict__ tw, int k, float2 (&w)[R])
+{
+    if constexpr ((R & (R - 1)) == 0) {
+        int row = 0;
+#pragma unroll
+        for (int p = 1; p < R; p <<= 1, row++) w[p] = tw[row * NS + k];
+    } else {
+#pragma unroll
+        for (int r = 1; r < R; r++) w[r] = tw[(r - 1) * NS + k];
+    }
+}
+
+// Completes w[1 .. R-1] from the loaded powers (power-of-two radices).
+template <int R>
+__device__ __forceinline__ void tw_expand(float2 (&w)[R])
+{
+    if constexpr ((R & (R - 1)) == 0) {
+#pragma unroll
+        for (int r = 3; r < R; r++) {
+            if ((r & (r - 1)) == 0) continue;
+            int hb = 1;
+            while (hb * 2 <= r) hb *= 2;
+            w[r] = cmul(w[hb], w[r - hb]);
+        }
+    }
+}
+
 // HBM endpoints of the direct passes: element j of this thread's transform.
 struct RowIo {  // contiguous transform
     const float2 *__restrict__ in;
@@ -123,11 +166,7 @@ __device__ __forceinline__ void pass_static(float *re, float *im, const float2 *
                     v[q][r] = make_float2(re[lbase + j], im[lbase + j]);
                 }
             }
-            if constexpr (NS > 1) {
-                const int k = b % NS;
-#pragma unroll
-                for (int r = 1; r < R; r++) w[q][r] = tw[(r - 1) * NS + k];
-            }
+            if constexpr (NS > 1) tw_load<R, NS>(tw, b % NS, w[q]);
         }
     }
     if constexpr (!GIN && !GOUT) __syncthreads();
@@ -136,6 +175,7 @@ __device__ __forceinline__ void pass_static(float *re, float *im, const float2 *
         const int b = t + q * T;
         if (active && (EXACT || b < NBF)) {
             if constexpr (NS > 1) {
+                tw_expand<R>(w[q]);
 #pragma unroll
                 for (int r = 1; r < R; r++) v[q][r] = cmul(v[q][r], w[q][r]);
             }
@@ -166,7 +206,7 @@ __device__ __forceinline__ void passes_static(float *re, float *im, const float2
     constexpr bool last = sizeof...(Rest) == 0;
     pass_static<L, T, S, NS, R, DIN && first, DOUT && last>(re, im, tw, lbase, t, active, io);
     if constexpr (!last)
-        passes_static<L, T, S, DIN, DOUT, NS * R, Rest...>(re, im, tw + NS * (R - 1), lbase, t, active, io);
+        passes_static<L, T, S, DIN, DOUT, NS * R, Rest...>(re, im, tw + NS * tw_rows<R>(), lbase, t, active, io);
 }
 
 // f = f0 + n0 * (f1 + n1 * f2)
@@ -215,9 +255,11 @@ __device__ __forceinline__ void split_chain(float *pl, const float2 *__restrict_
         const int b = t + q * T;
         if (active && (EXACT || b < NBF)) {
             if constexpr (NS > 1) {
-                const int k = b % NS;
+                float2 w[R];
+                tw_load<R, NS>(tw, b % NS, w);
+                tw_expand<R>(w);
 #pragma unroll
-                for (int r = 1; r < R; r++) v[q][r] = cmul(v[q][r], tw[(r - 1) * NS + k]);
+                for (int r = 1; r < R; r++) v[q][r] = cmul(v[q][r], w[r]);
             }
             dft<R, S>(v[q]);
         }
@@ -269,7 +311,7 @@ __device__ __forceinline__ void split_chain(float *pl, const float2 *__restrict_
                 }
             }
         }
-        split_chain<L, T, S, NS * R, Rest...>(pl, tw + NS * (R - 1), lbase, t, active, n, gout);
+        split_chain<L, T, S, NS * R, Rest...>(pl, tw + NS * tw_rows<R>(), lbase, t, active, n, gout);
     }
 }
 
