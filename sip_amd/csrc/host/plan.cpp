@@ -53,7 +53,7 @@ static float2 root_f(int64_t m, int64_t n, int sign)
 static TableRef pass_twiddles(const StockhamInstance *in, int sign)
 {
     TableRef t;
-    t.key = "twp:" + std::to_string(in->L);
+    t.key = "twk:" + std::to_string(in->L);
     for (int p = 0; p < in->npass; p++) t.key += "," + std::to_string(in->radix[p]);
     t.key += ":" + std::to_string(sign);
     t.gen = [in, sign]() {
@@ -63,12 +63,22 @@ static TableRef pass_twiddles(const StockhamInstance *in, int sign)
         for (int p = 0; p < in->npass; p++) {
             const int64_t R = in->radix[p];
             const int64_t step = L / (ns * R);
-            // rows [row][k] (lanes of a wave, consecutive k, read consecutive entries): r = 1, 2, 4,
-            // .. R/2 for a power-of-two radix (the kernel multiplies the others together), every
-            // r = 1 .. R-1 otherwise
-            const bool pow2 = (R & (R - 1)) == 0;
-            for (int64_t r = 1; r < R; r = pow2 ? r * 2 : r + 1)
-                for (int64_t k = 0; k < ns; k++) v.push_back(root_f(k * r * step, L, sign));
+            // power-of-two radix: k-major [k][W^1, W^2, W^4, W^8] (radix 8 pads the fourth slot,
+            // radix 4 stores two, radix 2 one and a pad; the kernel multiplies the other r
+            // together); other radices: row-major [r-1][k] over every r = 1 .. R-1. Every pass
+            // holds an even number of entries per k, so each pass table stays 16-byte aligned.
+            if ((R & (R - 1)) == 0) {
+                const int slots = R <= 4 ? 2 : 4;
+                if (R > 16) throw SipError(AsdSip::ACL_ERROR_INTERNAL_ERROR, "radix above 16 in a leaf");
+                for (int64_t k = 0; k < ns; k++)
+                    for (int sl = 0; sl < slots; sl++) {
+                        const int64_t r = int64_t(1) << sl;
+                        v.push_back(r < R ? root_f(k * r * step, L, sign) : float2{0.f, 0.f});
+                    }
+            } else {
+                for (int64_t r = 1; r < R; r++)
+                    for (int64_t k = 0; k < ns; k++) v.push_back(root_f(k * r * step, L, sign));
+            }
             ns *= R;
         }
         if (v.empty()) v.push_back(root_f(0, 1, 1));

@@ -131,7 +131,6 @@ __device__ __forceinline__ void pass_mfma16(float *re, float *im, const float2 *
     const int j = lane & 15, gq = lane >> 4;
     float xr[4][4], xi[4][4];
     int fls[4], bs[4];
-    static_assert(tw_rows<16>() == 4, "radix-16 twiddle rows are W^1, W^2, W^4, W^8");
     // gather: block u of this wave covers global butterflies (4 wave + u) * 16 + 0..15
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -147,8 +146,9 @@ __device__ __forceinline__ void pass_mfma16(float *re, float *im, const float2 *
         float2 lane_tw[4];
         if constexpr (NS > 1) {
             if (act) {
-                const int k = b % NS;
-                const float2 w1 = tw[k], w2 = tw[NS + k], w4 = tw[2 * NS + k], w8 = tw[3 * NS + k];
+                float2 wl[16];
+                tw_load<16, NS>(tw, b % NS, wl);
+                const float2 w1 = wl[1], w2 = wl[2], w4 = wl[4], w8 = wl[8];
                 const float2 w3 = cmul(w1, w2);
                 const float2 one = make_float2(1.f, 0.f);
                 const float2 hi = gq == 0 ? one : gq == 1 ? w4 : gq == 2 ? w8 : cmul(w4, w8);

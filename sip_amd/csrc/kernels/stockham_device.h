@@ -89,16 +89,20 @@ __device__ __forceinline__ int lds_idx_write(int hi, int k, int r)
 }
 
 // Twiddle rows of a radix-R pass. A power-of-two radix stores W^{k r} for r = 1, 2, 4, .. R/2 only
-// (the other r are products of those, at most three multiplies deep, so a pass loads log2(R)
-// twiddles instead of R - 1); other radices store every r = 1 .. R-1. Rows are [row][k], k < NS.
+// (the other r are products of those, at most three multiplies deep), k-major and padded to an
+// even count, so a radix-16 or radix-8 pass loads its twiddles with two 16-byte loads: layout
+// [k][4] (radix 8 and 16), [k][2] (radix 4; radix 2 uses the first slot). Other radices store
+// every r = 1 .. R-1 row-major, [r-1][k]. Every pass has an even number of rows, so every pass
+// table starts 16-byte aligned.
 template <int R>
 constexpr int tw_rows()
 {
-    if constexpr ((R & (R - 1)) == 0) {
-        int n = 0;
-        for (int p = 1; p < R; p <<= 1) n++;
-        return n;
+    if constexpr (R == 2 || R == 4) {
+        return 2;
+    } else if constexpr (R == 8 || R == 16) {
+        return 4;
     } else {
+        static_assert((R & (R - 1)) != 0, "power-of-two radices above 16 have no twiddle layout");
         return R - 1;
     }
 }
@@ -106,10 +110,19 @@ constexpr int tw_rows()
 template <int R, int NS>
 __device__ __forceinline__ void tw_load(const float2 *__restrict__ tw, int k, float2 (&w)[R])
 {
-    if constexpr ((R & (R - 1)) == 0) {
-        int row = 0;
-#pragma unroll
-        for (int p = 1; p < R; p <<= 1, row++) w[p] = tw[row * NS + k];
+    if constexpr (R == 2) {
+        w[1] = tw[2 * k];
+    } else if constexpr (R == 4) {
+        const float4 a = reinterpret_cast<const float4 *>(tw)[k];
+        w[1] = make_float2(a.x, a.y);
+        w[2] = make_float2(a.z, a.w);
+    } else if constexpr (R == 8 || R == 16) {
+        const float4 a = reinterpret_cast<const float4 *>(tw)[2 * k];
+        const float4 b = reinterpret_cast<const float4 *>(tw)[2 * k + 1];
+        w[1] = make_float2(a.x, a.y);
+        w[2] = make_float2(a.z, a.w);
+        w[4] = make_float2(b.x, b.y);
+        if constexpr (R == 16) w[8] = make_float2(b.z, b.w);
     } else {
 #pragma unroll
         for (int r = 1; r < R; r++) w[r] = tw[(r - 1) * NS + k];
