@@ -441,19 +441,19 @@ struct Builder {
         a.direct_in = (io_mode == IO_C2C || io_mode == IO_R2C_EVEN) && a.tin.es == 1 && !a.tin.unit_f && T >= 16;
         a.direct_out = (io_mode == IO_C2C || io_mode == IO_C2R_EVEN) && a.tout.es == 1 && !a.tout.unit_f &&
                        T >= 16 && tw4_ntot == 0;
-        // Measured on MI355X (N=4096 x 16384): the two-plane kernel in launch order beats the
-        // single-plane exchange kernel (twice the resident workgroups) and XCD-contiguous tile order;
-        // both stay available as tuning switches SIP_FFT_SPLIT_LDS=1 and SIP_FFT_XCD=1.
         const bool rows = io_mode == IO_C2C && a.direct_in && a.direct_out && inst->rows_fwd != nullptr;
-        // single-plane exchange by default where two planes leave room for only one workgroup per CU
-        // (16384: 479 us against 646 us per 1 GiB on MI355X)
-        const bool one_per_cu = (size_t)2 * F * lp * sizeof(float) > 80 * 1024;
         // column tiles (C2C, strided on both sides): first / last pass straight from / to HBM with
         // lanes walking adjacent columns
         a.col_direct = (io_mode == IO_C2C && a.tin.unit_f && a.tout.unit_f && !mf &&
                         env_flag("SIP_FFT_COL_DIRECT", true)) ? 1 : 0;
+        // Single-plane exchange (half the LDS, twice the resident workgroups) by default for rows
+        // with 32 or more threads per transform. Measured on MI355X, 1 GiB of C2C rows: 4096
+        // 373 us against 397 us, 1024 370 / 395, 8192 428 / 497, 16384 436 / 646; at 16 threads
+        // (length 256) the two-plane kernel stays ahead (371 / 381). Launch order beats
+        // XCD-contiguous tile order (4096 x 16384: 0.194 against 0.200 ms); the latter stays
+        // available as the tuning switch SIP_FFT_XCD=1.
         a.split_lds = rows && (p.tune.split_lds >= 0 ? p.tune.split_lds == 1
-                                                     : env_flag("SIP_FFT_SPLIT_LDS", one_per_cu));
+                                                     : env_flag("SIP_FFT_SPLIT_LDS", T >= 32));
         a.xcd_remap = p.tune.xcd >= 0 ? p.tune.xcd == 1 : env_flag("SIP_FFT_XCD", false);
         st.tw = pass_twiddles(inst, sign);
         if (mf) st.mfw = mfma_dft_table(p.precision, sign);

@@ -159,3 +159,33 @@ def test_inplace_c2c(torch_cuda):
     rel, _ = report("in-place 4096", xd.cpu().numpy(), oracle.c2c(x))
     h.destroy()
     assert rel <= TOL
+
+
+def test_unaligned_rows(torch_cuda):
+    """Rows 8-byte but not 16-byte aligned (views one element into a buffer): the direct passes
+    fall back to element-wise HBM access and give the same result as aligned rows."""
+    import torch
+    import sip_amd as S
+    from fftcase import make
+    n, b = 4096, 32
+    x = oracle.uniform_complex((b, n), 4343)
+    buf_in = torch.zeros(b * n + 1, dtype=torch.complex64, device="cuda")
+    buf_out = torch.zeros(b * n + 1, dtype=torch.complex64, device="cuda")
+    xin = buf_in[1:].view(b, n)
+    xin.copy_(torch.from_numpy(x).cuda())
+    yout = buf_out[1:].view(b, n)
+    assert xin.data_ptr() % 16 == 8 and yout.data_ptr() % 16 == 8
+    h = make([n], S.ASCEND_FFT_C2C, FWD, b)
+    try:
+        assert h.exec_c2c(xin, yout) == 0
+        assert h.synchronize() == 0
+        y_unaligned = yout.cpu().numpy()
+        xa = xin.contiguous().clone()
+        ya = torch.empty_like(xa)
+        assert h.exec_c2c(xa, ya) == 0
+        assert h.synchronize() == 0
+    finally:
+        h.destroy()
+    rel, _ = report("unaligned rows 4096x32", y_unaligned, oracle.c2c(x))
+    assert rel <= TOL
+    assert np.array_equal(y_unaligned, ya.cpu().numpy())
