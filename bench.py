@@ -13,8 +13,10 @@ profiles/traffic_c2c_4096x16384.json when present.
 
 cpu_baseline: the oracle's FP32 CPU FFT (same staged Stockham algorithm) on the host cores, median of
 runs over the full 16384 x 4096 batch; the leg also checks a sample of the GPU's output rows against
-the FP64 oracle. Secondary configurations (R2C/C2R 8192 x 4096, C2C 1536 / 2000 x 8192, C2C 1024 x 256)
-are timed the same way and reported under "secondary".
+the FP64 oracle. Secondary configurations (R2C/C2R 8192 x 4096, C2C 1536 / 2000 x 8192, C2C 1024 x 256,
+2D 1024^2 x 8; and 1 GiB arrays beyond the last-level cache: four-step 65536 x 2048, 2D 1024^2 x 128,
+3D 128^3 x 64) are timed the same way and reported under "secondary"; the precision, batching and
+serving forms have legs of their own.
 """
 from __future__ import annotations
 
@@ -34,8 +36,8 @@ PEAK_HBM_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-batching", action="store_true", help="skip the uneven-batch (batching form) leg")
@@ -320,7 +322,10 @@ def main():
                  ("c2c_2000x8192", [2000], S.ASCEND_FFT_C2C, 8192),
                  ("r2c_8192x4096", [8192], S.ASCEND_FFT_R2C, 4096),
                  ("c2r_8192x4096", [8192], S.ASCEND_FFT_C2R, 4096),
-                 ("c2c_2d_1024x1024x8", [1024, 1024], S.ASCEND_FFT_C2C, 8)]
+                 ("c2c_2d_1024x1024x8", [1024, 1024], S.ASCEND_FFT_C2C, 8),
+                 ("c2c_65536x2048", [65536], S.ASCEND_FFT_C2C, 2048),
+                 ("c2c_2d_1024x1024x128", [1024, 1024], S.ASCEND_FFT_C2C, 128),
+                 ("c2c_3d_128x128x128x64", [128, 128, 128], S.ASCEND_FFT_C2C, 64)]
         for name, dims, ft, b in cases:
             nn = int(math.prod(dims))
             with torch.cuda.stream(stream):
