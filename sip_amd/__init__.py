@@ -253,6 +253,15 @@ class FftHandle:
     def exec_c2r(self, x, y):
         return self._exec(lib().asdFftExecC2R, x, y)
 
+    def exec_c2c_separated(self, x_re, x_im, y_re, y_im):
+        """asdFftExecC2CSeparated: float32 real / imaginary planes of one shape each."""
+        ts = [AclTensor.from_torch(t) for t in (x_re, x_im, y_re, y_im)]
+        try:
+            return lib().asdFftExecC2CSeparated(self.h, *[ctypes.c_void_p(t.ptr) for t in ts])
+        finally:
+            for t in ts:
+                t.close()
+
     def synchronize(self):
         return lib().asdFftSynchronize(self.h)
 

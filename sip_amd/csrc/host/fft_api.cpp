@@ -61,10 +61,6 @@ int32_t finish_plan(Plan &p)
     p.failed = false;
     p.stages.clear();
     p.ref_steps = sip::reference_steps(p.fftType, p.direction, p.sizes, p.batch, p.dimType);
-    if (p.fftType == sip::kTypeC2CSep) {
-        p.failed = true;
-        return E::ACL_ERROR_API_NOT_SUPPORT;
-    }
     try {
         sip::plan_build(p);
         sip::plan_materialize(p);
@@ -292,15 +288,29 @@ int32_t asdFftExecC2CSeparated(void *handle, const aclTensor *inputReal, const a
         Plan *p = lookup(handle);
         if (!p) return E::ACL_ERROR_INVALID_PARAM;
         if (!inputReal || !inputImag || !outputReal || !outputImag) return E::ACL_ERROR_INTERNAL_ERROR;
-        // Same shape rule as the reference (fft_api.cpp:1236-1342); the split-plane kernels are
-        // not part of this library.
+        // Same shape rule as the reference (fft_api.cpp:1283-1342): the four planes have one shape.
         const aclTensor *ts[4] = {inputReal, inputImag, outputReal, outputImag};
         for (int i = 1; i < 4; i++) {
             if (ts[i]->ndim != ts[0]->ndim) return E::ACL_ERROR_INTERNAL_ERROR;
             for (int d = 0; d < ts[0]->ndim; d++)
                 if (ts[i]->dims[d] != ts[0]->dims[d]) return E::ACL_ERROR_INTERNAL_ERROR;
         }
-        return E::ACL_ERROR_API_NOT_SUPPORT;
+        std::lock_guard<std::mutex> lk(p->mu);
+        // the reference runs any plan here; only a C2C_SEP plan has split-plane stages
+        if (p->fftType != sip::kTypeC2CSep) return E::ACL_ERROR_INVALID_PARAM;
+        for (const aclTensor *t : ts)
+            if (!match(*p, t, ACL_FLOAT, false)) return E::ACL_ERROR_INVALID_PARAM;
+        if (!p->initialized) return E::ACL_ERROR_INVALID_PARAM;
+        if (p->ws_bytes > 0 && p->workspace == nullptr) return E::ACL_ERROR_INTERNAL_ERROR;
+        void *d[4];
+        for (int i = 0; i < 4; i++) {
+            d[i] = aclGetTensorData(ts[i]);
+            if (d[i] == nullptr) return E::ACL_ERROR_INTERNAL_ERROR;
+        }
+        int cur = 0;
+        if (hipGetDevice(&cur) == hipSuccess && cur != p->device) SIP_HIP_CHECK(hipSetDevice(p->device));
+        sip::plan_execute_sep(*p, d[0], d[1], d[2], d[3]);
+        return E::ACL_SUCCESS;
     });
 }
 
@@ -401,7 +411,7 @@ int32_t sipFftPlanDescribeDry(int32_t ndim, const int64_t *dims, int32_t fftType
             p.sizes.push_back(dims[i]);
         }
         p.ref_steps = sip::reference_steps(p.fftType, p.direction, p.sizes, p.batch, p.dimType);
-        if (fftType != sip::kTypeC2CSep) sip::plan_build(p);
+        sip::plan_build(p);
         return write_str(sip::plan_describe(p), buf, cap, needed);
     });
 }

@@ -329,7 +329,11 @@ constexpr int64_t kRowLeafMax = 16384;
 struct Builder {
     Plan &p;
     int64_t ws_count = 0;
+    // C2C_SEP plans: the user's input and output (BUF_IN / BUF_OUT) are two float planes each
+    bool sep = false;
     explicit Builder(Plan &pl) : p(pl) {}
+
+    bool is_user(const BufRef &b) const { return sep && (b.kind == BUF_IN || b.kind == BUF_OUT); }
 
     BufRef new_ws(size_t bytes)
     {
@@ -361,7 +365,10 @@ struct Builder {
     {
         const StockhamInstance *inst = find_instance(L);
         // precision form: C2C launches with radix-16 stages run them on the matrix cores
-        const StockhamInstance *mf = (p.precision != PREC_FP32 && io_mode == IO_C2C)
+        const bool in_sep = is_user(in.buf), out_sep = is_user(out.buf);
+        if ((in_sep || out_sep) && io_mode != IO_C2C)
+            throw SipError(AsdSip::ACL_ERROR_INTERNAL_ERROR, "split planes on a real-transform stage");
+        const StockhamInstance *mf = (p.precision != PREC_FP32 && io_mode == IO_C2C && !in_sep && !out_sep)
                                          ? find_mfma_instance(L, p.precision) : nullptr;
         if (mf) inst = mf;
         if (!inst) throw SipError(AsdSip::ACL_ERROR_API_NOT_SUPPORT, "no kernel for length " + std::to_string(L));
@@ -441,10 +448,21 @@ struct Builder {
         a.direct_in = (io_mode == IO_C2C || io_mode == IO_R2C_EVEN) && a.tin.es == 1 && !a.tin.unit_f && T >= 16;
         a.direct_out = (io_mode == IO_C2C || io_mode == IO_C2R_EVEN) && a.tout.es == 1 && !a.tout.unit_f &&
                        T >= 16 && tw4_ntot == 0;
+        // split-plane user buffers go through the LDS-staged load / store
+        a.in_sep = in_sep ? 1 : 0;
+        a.out_sep = out_sep ? 1 : 0;
+        if (in_sep) {
+            a.direct_in = 0;
+            a.tin.vec2 = 0;
+        }
+        if (out_sep) {
+            a.direct_out = 0;
+            a.tout.vec2 = 0;
+        }
         const bool rows = io_mode == IO_C2C && a.direct_in && a.direct_out && inst->rows_fwd != nullptr;
         // column tiles (C2C, strided on both sides): first / last pass straight from / to HBM with
         // lanes walking adjacent columns
-        a.col_direct = (io_mode == IO_C2C && a.tin.unit_f && a.tout.unit_f && !mf &&
+        a.col_direct = (io_mode == IO_C2C && a.tin.unit_f && a.tout.unit_f && !mf && !in_sep && !out_sep &&
                         env_flag("SIP_FFT_COL_DIRECT", true)) ? 1 : 0;
         // Single-plane exchange (half the LDS, twice the resident workgroups) by default for rows
         // with 32 or more threads per transform. Measured on MI355X, 1 GiB of C2C rows: 4096
@@ -497,6 +515,11 @@ struct Builder {
         x.n = n;
         x.out_len = out_len;
         x.scale = scale;
+        x.in_sep = is_user(in.buf) ? 1 : 0;
+        x.out_sep = is_user(out.buf) ? 1 : 0;
+        if ((x.in_sep && mode != AUX_CPLX_TRUNC && mode != AUX_BLUE_PRE) ||
+            (x.out_sep && mode != AUX_CPLX_TRUNC && mode != AUX_BLUE_POST))
+            throw SipError(AsdSip::ACL_ERROR_INTERNAL_ERROR, "split planes on an unsupported auxiliary stage");
         st.tab0 = tab;
         st.note = note;
         p.stages.push_back(st);
@@ -696,6 +719,7 @@ void plan_build(Plan &p)
     p.ws_offsets.clear();
     p.ws_bytes = 0;
     Builder b(p);
+    b.sep = p.fftType == kTypeC2CSep;
     const int sign = p.sign();
     const int nd = (int)p.sizes.size();
     BufRef IN, OUT;
@@ -705,7 +729,7 @@ void plan_build(Plan &p)
 
     if (nd == 1) {
         const int64_t N = p.sizes[0];
-        if (p.fftType == kTypeC2C) {
+        if (p.fftType == kTypeC2C || p.fftType == kTypeC2CSep) {
             if (p.dimType == kDimVertical) {
                 // (N, batch): axis 0 with element stride batch
                 Lay in{IN, B, {{B, 1}}};
@@ -728,7 +752,7 @@ void plan_build(Plan &p)
 
     // 2D / 3D: array [B][d0][d1]([d2]), last axis fastest.
     std::vector<int64_t> d = p.sizes;
-    if (p.fftType == kTypeC2C) {
+    if (p.fftType == kTypeC2C || p.fftType == kTypeC2CSep) {
         // last axis: rows in -> out, then the other axes in place on out
         int64_t rows = B;
         for (int i = 0; i < nd - 1; i++) rows *= d[i];
@@ -920,8 +944,16 @@ static void *resolve(const Plan &p, const BufRef &b, const void *in, void *out)
     return static_cast<char *>(p.workspace) + p.ws_offsets[(size_t)b.ws_index];
 }
 
-void plan_execute(Plan &p, const void *in, void *out)
+void plan_execute(Plan &p, const void *in, void *out) { plan_execute_sep(p, in, nullptr, out, nullptr); }
+
+void plan_execute_sep(Plan &p, const void *in, const void *in_im, void *out, void *out_im)
 {
+    // imaginary planes of the user buffers (split-plane plans only)
+    auto im_of = [&](const BufRef &b) -> float * {
+        if (b.kind == BUF_IN) return static_cast<float *>(const_cast<void *>(in_im));
+        if (b.kind == BUF_OUT) return static_cast<float *>(out_im);
+        return nullptr;
+    };
     for (Stage &s : p.stages) {
         void *src = resolve(p, s.in, in, out);
         void *dst = resolve(p, s.out, in, out);
@@ -929,6 +961,10 @@ void plan_execute(Plan &p, const void *in, void *out)
             StockhamArgs a = s.sa;
             a.in = src;
             a.out = dst;
+            a.in_im = im_of(s.in);
+            a.out_im = im_of(s.out);
+            if ((a.in_sep && !a.in_im) || (a.out_sep && !a.out_im))
+                throw SipError(AsdSip::ACL_ERROR_INTERNAL_ERROR, "split-plane plan executed without imaginary planes");
             if (a.tin.vec2 && (reinterpret_cast<uintptr_t>(src) & 15)) a.tin.vec2 = 0;
             if (a.tout.vec2 && (reinterpret_cast<uintptr_t>(dst) & 15)) a.tout.vec2 = 0;
             void *args[] = {&a};
@@ -938,6 +974,10 @@ void plan_execute(Plan &p, const void *in, void *out)
             AuxArgs a = s.xa;
             a.in = src;
             a.out = dst;
+            a.in_im = im_of(s.in);
+            a.out_im = im_of(s.out);
+            if ((a.in_sep && !a.in_im) || (a.out_sep && !a.out_im))
+                throw SipError(AsdSip::ACL_ERROR_INTERNAL_ERROR, "split-plane plan executed without imaginary planes");
             SIP_HIP_CHECK(aux_launch(a, p.stream));
         }
     }
@@ -969,7 +1009,8 @@ std::string plan_describe(const Plan &p)
                << ",\"grid\":" << s.grid.x << ",\"block\":" << s.block.x << ",\"unit_f_in\":" << s.sa.tin.unit_f
                << ",\"unit_f_out\":" << s.sa.tout.unit_f << ",\"tw4\":" << s.sa.tw4_ntot
                << ",\"direct_in\":" << s.sa.direct_in << ",\"direct_out\":" << s.sa.direct_out
-               << ",\"split_lds\":" << s.sa.split_lds << ",\"col_direct\":" << s.sa.col_direct << ",\"mfma\":" << (s.mfw.used() ? 1 : 0)
+               << ",\"split_lds\":" << s.sa.split_lds << ",\"col_direct\":" << s.sa.col_direct
+               << ",\"in_sep\":" << s.sa.in_sep << ",\"out_sep\":" << s.sa.out_sep << ",\"mfma\":" << (s.mfw.used() ? 1 : 0)
                << ",\"xcd_remap\":" << s.sa.xcd_remap;
         } else {
             os << ",\"mode\":" << s.xa.mode << ",\"rows\":" << s.xa.rows << ",\"n\":" << s.xa.n

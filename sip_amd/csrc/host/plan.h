@@ -134,6 +134,8 @@ void plan_build(Plan &p);
 void plan_materialize(Plan &p);
 // Enqueues the stages on p.stream. in/out are device addresses of the user tensors.
 void plan_execute(Plan &p, const void *in, void *out);
+// Split-plane (C2C_SEP) execution: user buffers are real / imaginary float planes.
+void plan_execute_sep(Plan &p, const void *in, const void *in_im, void *out, void *out_im);
 // JSON description of the stages.
 std::string plan_describe(const Plan &p);
 

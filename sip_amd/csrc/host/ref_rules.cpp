@@ -167,10 +167,12 @@ std::vector<int64_t> fftb_radix(int64_t n)
 
 std::string core_step(int type, int64_t n, int64_t stride, int64_t batch, bool forward)
 {
-    (void)forward;
     std::ostringstream os;
     std::string core;
-    if (type == kTypeC2C && stride > 1) {
+    if (type == kTypeC2CSep) {
+        // split planes: FFT-B for forward 256 / 512, one-GEMM DFT otherwise (fft_api.cpp:394-399)
+        core = ((n == 256 || n == 512) && forward) ? "FftBSep" : "DftSep";
+    } else if (type == kTypeC2C && stride > 1) {
         core = "FftStride";
     } else {
         if (type == kTypeC2C) core = find_config(batch, n);
@@ -237,6 +239,13 @@ std::vector<std::string> reference_steps(int type, int direction, const std::vec
     if (sizes.size() == 1) {
         const int64_t stride = (dimType == kDimVertical) ? batch : 1;
         s.push_back(core_step(type, sizes[0], stride, batch, fwd));
+        return s;
+    }
+    if (sizes.size() == 2 && type == kTypeC2CSep) return s;  // init2DSteps adds no step for C2C_SEP
+    if (sizes.size() == 3 && type == kTypeC2CSep) {
+        // one fused 3D split-plane DFT step (fft_api.cpp:451-458, 657-660)
+        s.push_back("DddSep(X=" + std::to_string(sizes[0]) + ",Y=" + std::to_string(sizes[1]) + ",Z=" +
+                    std::to_string(sizes[2]) + ",batch=" + std::to_string(batch) + ")");
         return s;
     }
     if (sizes.size() == 2) {

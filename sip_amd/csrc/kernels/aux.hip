@@ -17,6 +17,22 @@ __device__ __forceinline__ int64_t row_base(uint32_t r, const AuxRows &d)
     return (int64_t)r0 * d.s0 + (int64_t)r1 * d.s1;
 }
 
+// Complex element at offset o of the input / output, interleaved or split into two planes.
+__device__ __forceinline__ float2 cin_at(const AuxArgs &a, int64_t o)
+{
+    if (a.in_sep) return make_float2(static_cast<const float *>(a.in)[o], a.in_im[o]);
+    return static_cast<const float2 *>(a.in)[o];
+}
+__device__ __forceinline__ void cout_at(const AuxArgs &a, int64_t o, float2 v)
+{
+    if (a.out_sep) {
+        static_cast<float *>(a.out)[o] = v.x;
+        a.out_im[o] = v.y;
+    } else {
+        static_cast<float2 *>(a.out)[o] = v;
+    }
+}
+
 __global__ __launch_bounds__(256) void aux_kernel(const AuxArgs a)
 {
     const int64_t total = a.rows * a.out_len;
@@ -35,7 +51,7 @@ __global__ __launch_bounds__(256) void aux_kernel(const AuxArgs a)
                 break;
             }
             case AUX_CPLX_TRUNC:
-                cout[ob + k * a.rout.es] = cin[ib + k * a.rin.es];
+                cout_at(a, ob + k * a.rout.es, cin_at(a, ib + k * a.rin.es));
                 break;
             case AUX_HERM_EXTEND: {
                 const int64_t h = a.n / 2;
@@ -77,13 +93,13 @@ __global__ __launch_bounds__(256) void aux_kernel(const AuxArgs a)
                 break;
             }
             case AUX_BLUE_PRE:
-                cout[ob + k * a.rout.es] = (k < a.n) ? cmul(cin[ib + k * a.rin.es], a.tab0[k]) : make_float2(0.f, 0.f);
+                cout[ob + k * a.rout.es] = (k < a.n) ? cmul(cin_at(a, ib + k * a.rin.es), a.tab0[k]) : make_float2(0.f, 0.f);
                 break;
             case AUX_BLUE_MUL:
                 cout[ob + k * a.rout.es] = cmul(cin[ib + k * a.rin.es], a.tab0[k]);
                 break;
             case AUX_BLUE_POST:
-                cout[ob + k * a.rout.es] = cscale(cmul(cin[ib + k * a.rin.es], a.tab0[k]), a.scale);
+                cout_at(a, ob + k * a.rout.es, cscale(cmul(cin[ib + k * a.rin.es], a.tab0[k]), a.scale));
                 break;
             default:
                 break;

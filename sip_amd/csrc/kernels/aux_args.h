@@ -30,6 +30,9 @@ struct AuxRows {
 struct AuxArgs {
     const void *in;
     void *out;
+    const float *in_im;   // split-plane input (C2C modes): real parts at (float *)in
+    float *out_im;        // split-plane output: real parts at (float *)out
+    int32_t in_sep, out_sep;
     const float2 *tab0;
     int64_t rows;
     int64_t n;          // logical transform length

@@ -69,6 +69,10 @@ struct StockhamArgs {
     int32_t col_fixed;     // column tiles: block size is a multiple of F (one transform per thread)
     int32_t col_direct;    // column tile (C2C, strided in and out): lanes walk transforms, the first
                            // pass reads and the last pass writes HBM directly
+    const float *in_im;    // split-plane input: real parts at (float *)in, imaginary parts here
+    float *out_im;         // split-plane output: real parts at (float *)out, imaginary parts here
+    int32_t in_sep;        // the input is two float planes (element offsets as for complex)
+    int32_t out_sep;       // the output is two float planes
     FastDiv div_in_len;    // divides by in_len
     FastDiv div_out_len;   // divides by out_len
     FastDiv div_F;         // divides by F

@@ -451,6 +451,25 @@ __device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds,
     const int in_len = a.in_len;
     if (a.direct_in || a.col_direct) {
         // the first radix pass reads HBM directly
+    } else if (a.in_sep) {
+        // split planes (C2C): real and imaginary parts from two float arrays
+        const float *sre = static_cast<const float *>(a.in), *sim = a.in_im;
+        const int total = F * in_len;
+        for (int i = tid; i < total; i += nthreads) {
+            int fl, j;
+            if (a.tin.unit_f) {
+                j = (int)fdiv((uint32_t)i, a.div_F);
+                fl = i - j * F;
+            } else {
+                fl = (int)fdiv((uint32_t)i, a.div_in_len);
+                j = i - fl * in_len;
+            }
+            if (fl < nf) {
+                const int64_t o = in_base[fl] + (int64_t)j * a.tin.es;
+                re[fl * LP + lds_pad(j)] = sre[o];
+                im[fl * LP + lds_pad(j)] = sim[o];
+            }
+        }
     } else if (a.io_mode == IO_R2C_FULL) {
         const float *src = static_cast<const float *>(a.in);
         const int total = F * in_len;
@@ -616,7 +635,27 @@ __device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds,
 
     // ---------------------------------------------------------------- store tile
     const int out_len = a.out_len;
-    if (a.io_mode == IO_R2C_EVEN) {
+    if (a.out_sep) {
+        // split planes (C2C), with the four-step twiddle when the stage carries one
+        float *dre = static_cast<float *>(a.out), *dim = a.out_im;
+        for (int i = tid; i < F * out_len; i += nthreads) {
+            int fl, j;
+            if (a.tout.unit_f) {
+                j = (int)fdiv((uint32_t)i, a.div_F);
+                fl = i - j * F;
+            } else {
+                fl = (int)fdiv((uint32_t)i, a.div_out_len);
+                j = i - fl * out_len;
+            }
+            if (fl < nf) {
+                float2 v = make_float2(re[fl * LP + lds_pad(j)], im[fl * LP + lds_pad(j)]);
+                if (a.tw4_ntot != 0) v = cmul(v, tw4_root(a, col[fl] * (int64_t)j));
+                const int64_t o = out_base[fl] + (int64_t)j * a.tout.es;
+                dre[o] = v.x;
+                dim[o] = v.y;
+            }
+        }
+    } else if (a.io_mode == IO_R2C_EVEN) {
         // X[k] = Z[k] (1 - i w^k)/2 + conj Z[L-k] (1 + i w^k)/2, k = 0..L
         float2 *dst = static_cast<float2 *>(a.out);
         for (int i = tid; i < F * out_len; i += nthreads) {

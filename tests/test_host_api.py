@@ -78,8 +78,8 @@ def test_exec_on_uninitialised_plan_and_bad_tensors():
     one = S.AclTensor(0x1000, [1], S.ACL_COMPLEX64)
     assert L.asdFftExecC2C(h.h, ctypes.c_void_p(one.ptr), ctypes.c_void_p(one.ptr)) == INV
     assert L.asdFftExecC2C(h.h, None, None) == INV
-    # separated and ISTFT entry points exist and answer "not supported"
-    assert L.asdFftExecC2CSeparated(h.h, one.ptr, one.ptr, one.ptr, one.ptr) == UNSUP
+    # separated exec on a plan that is not C2C_SEP: invalid parameter; ISTFT answers "not supported"
+    assert L.asdFftExecC2CSeparated(h.h, one.ptr, one.ptr, one.ptr, one.ptr) == INV
     assert L.asdFftIstftMakePlan(h.h, one.ptr, 16, 4, 16, True, False, False, 0, True) == UNSUP
     assert L.asdFftExecIstft(h.h, one.ptr, None, one.ptr) == UNSUP
     one.close()
