@@ -12,7 +12,9 @@ forward uses e^{-2 pi i kn/N}, inverse e^{+2 pi i kn/N} with no 1/N scaling; R2C
 transforms are separable, R2C doing the real axis (last) first and C2R last.
 
 Every function here is pinned by tests/test_oracle.py against closed forms (impulse, constant,
-pure tones, the ramp the reference's examples feed in), numpy.fft, and the brute-force DFT.
+pure tones, the ramp the reference's examples feed in), numpy.fft, and the brute-force DFT; the
+ISTFT restatement against torch.istft (an independent implementation of the same definition), STFT
+round trips and a constant-frame closed form.
 """
 from __future__ import annotations
 
@@ -134,6 +136,36 @@ def c2c_nd(x: np.ndarray, ndim: int, sign: int = FORWARD) -> np.ndarray:
     for ax in range(a.ndim - ndim, a.ndim):
         a = c2c_axis(a, ax, sign)
     return a
+
+
+def istft_out_len(n_fft: int, hop: int, n_frames: int) -> int:
+    """Output length of the reference's ISTFT with center=True and length 0: the overlap-add span
+    n_fft + hop (n_frames - 1) minus n_fft/2 at each end (core/fft/istft.cpp:276-293)."""
+    return n_fft + hop * (n_frames - 1) - 2 * (n_fft // 2)
+
+
+def istft(x: np.ndarray, n_fft: int, hop: int, window: np.ndarray) -> np.ndarray:
+    """ISTFT with the only parameters the reference runs (core/fft/istft.cpp:305-410 forces
+    center=True, normalized=False, onesided=False, return_complex=True, length=0), in FP64.
+
+    x: (channel, n_fft, n_frames) complex spectra. Per core/fft/fftcore/fft_core_istft_any.cpp:
+    292-380: frames = IDFT_{n_fft}(x[c, :, t]) / n_fft; y[c, t hop + n] += frames[c, t, n] w[n];
+    env[t hop + n] += w[n]^2; y / env, then the samples [n_fft/2, end - n_fft/2) are kept.
+    Returns (channel, istft_out_len) complex128."""
+    a = np.asarray(x, dtype=np.complex128)
+    ch, nf, nfr = a.shape
+    assert nf == n_fft
+    w = np.asarray(window, dtype=np.float64)
+    frames = c2c_axis(a, 1, INVERSE) / n_fft          # (ch, n_fft, n_frames)
+    total = n_fft + hop * (nfr - 1)
+    y = np.zeros((ch, total), dtype=np.complex128)
+    env = np.zeros(total, dtype=np.float64)
+    for t in range(nfr):
+        y[:, t * hop:t * hop + n_fft] += frames[:, :, t] * w
+        env[t * hop:t * hop + n_fft] += w * w
+    start = n_fft // 2
+    end = total - n_fft // 2
+    return y[:, start:end] / env[start:end]
 
 
 def r2c(x: np.ndarray, sign: int = FORWARD) -> np.ndarray:

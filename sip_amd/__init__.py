@@ -253,6 +253,25 @@ class FftHandle:
     def exec_c2r(self, x, y):
         return self._exec(lib().asdFftExecC2R, x, y)
 
+    def istft_make_plan(self, x, n_fft, hop, win_length, center=True, normalized=False, onesided=False, length=0,
+                        return_complex=True):
+        """asdFftIstftMakePlan for the spectra tensor x (channel, n_fft, frames), complex64."""
+        t = AclTensor.from_torch(x)
+        try:
+            return lib().asdFftIstftMakePlan(self.h, ctypes.c_void_p(t.ptr), n_fft, hop, win_length, center,
+                                             normalized, onesided, length, return_complex)
+        finally:
+            t.close()
+
+    def exec_istft(self, x, window, y):
+        """asdFftExecIstft: spectra x, float window (n_fft), complex output (channel, out_len)."""
+        ts = [AclTensor.from_torch(t) for t in (x, window, y)]
+        try:
+            return lib().asdFftExecIstft(self.h, *[ctypes.c_void_p(t.ptr) for t in ts])
+        finally:
+            for t in ts:
+                t.close()
+
     def exec_c2c_separated(self, x_re, x_im, y_re, y_im):
         """asdFftExecC2CSeparated: float32 real / imaginary planes of one shape each."""
         ts = [AclTensor.from_torch(t) for t in (x_re, x_im, y_re, y_im)]

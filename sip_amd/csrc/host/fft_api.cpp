@@ -60,7 +60,8 @@ int32_t finish_plan(Plan &p)
     p.initialized = false;
     p.failed = false;
     p.stages.clear();
-    p.ref_steps = sip::reference_steps(p.fftType, p.direction, p.sizes, p.batch, p.dimType);
+    p.ref_steps = p.istft.active ? sip::reference_istft_steps(p.istft.channel, p.istft.nfft, p.istft.frames, p.istft.hop)
+                                 : sip::reference_steps(p.fftType, p.direction, p.sizes, p.batch, p.dimType);
     try {
         sip::plan_build(p);
         sip::plan_materialize(p);
@@ -162,6 +163,7 @@ int32_t asdFftMakePlan1D(void *handle, int64_t fftSize, int32_t fftType, int32_t
         if (batchSize <= 0) return E::ACL_ERROR_INVALID_PARAM;
         if (!valid_type(fftType) || !valid_dir(direction)) return E::ACL_ERROR_INVALID_PARAM;
         std::lock_guard<std::mutex> lk(p->mu);
+        p->istft = Plan::Istft{};
         p->fftType = fftType;
         p->direction = direction;
         p->sizes = {fftSize};
@@ -190,6 +192,7 @@ int32_t asdFftMakePlan2D(void *handle, int64_t fftSizeX, int64_t fftSizeY, int32
         if (!valid_type(fftType) || !valid_dir(direction)) return E::ACL_ERROR_INVALID_PARAM;
         Plan *p = lookup(handle);
         std::lock_guard<std::mutex> lk(p->mu);
+        p->istft = Plan::Istft{};
         p->fftType = fftType;
         p->direction = direction;
         p->batch = batchSize;
@@ -210,6 +213,7 @@ int32_t asdFftMakePlan3D(void *handle, int64_t fftSizeX, int64_t fftSizeY, int64
         if (!valid_type(fftType) || !valid_dir(direction)) return E::ACL_ERROR_INVALID_PARAM;
         Plan *p = lookup(handle);
         std::lock_guard<std::mutex> lk(p->mu);
+        p->istft = Plan::Istft{};
         p->fftType = fftType;
         p->direction = direction;
         p->batch = batchSize;
@@ -248,7 +252,7 @@ int32_t asdFftExecC2C(void *handle, const aclTensor *input, const aclTensor *out
         Plan *p = lookup(handle);
         if (!p) return E::ACL_ERROR_INVALID_PARAM;
         std::lock_guard<std::mutex> lk(p->mu);
-        if (p->fftType != sip::kTypeC2C) return E::ACL_ERROR_INVALID_PARAM;
+        if (p->fftType != sip::kTypeC2C || p->istft.active) return E::ACL_ERROR_INVALID_PARAM;
         if (!match(*p, input, ACL_COMPLEX64, false)) return E::ACL_ERROR_INVALID_PARAM;
         if (!match(*p, output, ACL_COMPLEX64, false)) return E::ACL_ERROR_INVALID_PARAM;
         return exec_common(*p, input, output);
@@ -314,21 +318,85 @@ int32_t asdFftExecC2CSeparated(void *handle, const aclTensor *inputReal, const a
     });
 }
 
+// ISTFT (core/fft/istft.cpp:305-575). The reference runs one parameter set only and forces it,
+// with a warning, whatever the caller passes: center = true, normalized = false, onesided = false,
+// return_complex = true, length = 0 (istft.cpp:330-370); this library does the same, so a caller
+// gets the reference's result. Checks and status codes follow IstftParamsCheck and
+// asdFftExecIstft: input complex64 (channel, n_fft, frames) with frames > 1, 0 < hop <= win_length,
+// win_length == n_fft, n_fft < 1500, hop < 1500 (100000 otherwise); window float (n_fft);
+// output complex64 (channel, n_fft + hop (frames - 1) - 2 floor(n_fft / 2)).
 int32_t asdFftIstftMakePlan(void *handle, const aclTensor *input, int64_t nFft, int64_t hopLengthOpt,
                             int64_t winLengthOpt, bool center, bool normalized, bool onesidedOpt, int64_t lengthOpt,
                             bool returnComplex)
 {
-    (void)input; (void)nFft; (void)hopLengthOpt; (void)winLengthOpt; (void)center; (void)normalized;
-    (void)onesidedOpt; (void)lengthOpt; (void)returnComplex;
-    if (!lookup(handle)) return E::ACL_ERROR_INVALID_PARAM;
-    return E::ACL_ERROR_API_NOT_SUPPORT;
+    return guarded([&]() -> int32_t {
+        Plan *p = lookup(handle);
+        if (!p) return E::ACL_ERROR_INVALID_PARAM;
+        if (input == nullptr || input->dtype != ACL_COMPLEX64 || input->ndim != 3) return E::ACL_ERROR_INVALID_PARAM;
+        const int64_t channel = input->dims[0], fftSize = input->dims[1], frames = input->dims[2];
+        if (!center) SIP_LOG(WARN, "istft: only center = true is supported; using it");
+        if (normalized) SIP_LOG(WARN, "istft: only normalized = false is supported; using it");
+        if (onesidedOpt) SIP_LOG(WARN, "istft: only onesided = false is supported; using it");
+        if (!returnComplex) SIP_LOG(WARN, "istft: only return_complex = true is supported; using it");
+        if (lengthOpt != 0) SIP_LOG(WARN, "istft: only length = 0 is supported; using it");
+        if (frames <= 1) return E::ACL_ERROR_INVALID_PARAM;
+        if (nFft != fftSize) return E::ACL_ERROR_INVALID_PARAM;
+        if (!(hopLengthOpt > 0 && hopLengthOpt <= winLengthOpt)) return E::ACL_ERROR_INVALID_PARAM;
+        if (!(winLengthOpt > 0 && winLengthOpt == nFft)) return E::ACL_ERROR_INVALID_PARAM;
+        if (nFft >= 1500 || hopLengthOpt >= 1500) return E::ACL_ERROR_INVALID_PARAM;
+        if (channel <= 0) return E::ACL_ERROR_INVALID_PARAM;
+        std::lock_guard<std::mutex> lk(p->mu);
+        p->fftType = sip::kTypeC2C;
+        p->direction = sip::kDirInverse;
+        p->dimType = sip::kDimHorizontal;
+        p->sizes = {nFft};
+        p->strides = {1};
+        p->batch = channel * frames;
+        p->istft.active = true;
+        p->istft.channel = channel;
+        p->istft.nfft = nFft;
+        p->istft.frames = frames;
+        p->istft.hop = hopLengthOpt;
+        p->istft.win_len = winLengthOpt;
+        p->istft.out_len = nFft + hopLengthOpt * (frames - 1) - 2 * (nFft / 2);
+        const int32_t st = finish_plan(*p);
+        if (st != E::ACL_SUCCESS) {
+            p->istft = Plan::Istft{};
+            return E::ACL_ERROR_INTERNAL_ERROR;  // the reference's MakePlan1DFft failure
+        }
+        return st;
+    });
 }
 
 int32_t asdFftExecIstft(void *handle, const aclTensor *input, const aclTensor *windowOpt, const aclTensor *output)
 {
-    (void)input; (void)windowOpt; (void)output;
-    if (!lookup(handle)) return E::ACL_ERROR_INVALID_PARAM;
-    return E::ACL_ERROR_API_NOT_SUPPORT;
+    return guarded([&]() -> int32_t {
+        Plan *p = lookup(handle);
+        if (!p) return E::ACL_ERROR_INVALID_PARAM;
+        std::lock_guard<std::mutex> lk(p->mu);
+        const Plan::Istft &q = p->istft;
+        if (!q.active) return E::ACL_ERROR_INVALID_PARAM;  // the plan holds no ISTFT description
+        if (input == nullptr || input->dtype != ACL_COMPLEX64 || input->ndim != 3 || !contiguous(input))
+            return E::ACL_ERROR_INVALID_PARAM;
+        if (input->dims[0] != q.channel || input->dims[1] != q.nfft || input->dims[2] != q.frames)
+            return E::ACL_ERROR_INVALID_PARAM;
+        if (output == nullptr || output->dtype != ACL_COMPLEX64 || output->ndim != 2 || !contiguous(output))
+            return E::ACL_ERROR_INVALID_PARAM;
+        if (output->dims[0] != q.channel || output->dims[1] != q.out_len) return E::ACL_ERROR_INVALID_PARAM;
+        if (windowOpt == nullptr || windowOpt->dtype != ACL_FLOAT || windowOpt->ndim != 1 || !contiguous(windowOpt))
+            return E::ACL_ERROR_INVALID_PARAM;
+        if (windowOpt->dims[0] != q.win_len || windowOpt->dims[0] != q.nfft) return E::ACL_ERROR_INVALID_PARAM;
+        if (!p->initialized) return E::ACL_ERROR_INVALID_PARAM;
+        if (p->ws_bytes > 0 && p->workspace == nullptr) return E::ACL_ERROR_INTERNAL_ERROR;
+        const void *src = aclGetTensorData(input);
+        const void *win = aclGetTensorData(windowOpt);
+        void *dst = aclGetTensorData(output);
+        if (src == nullptr || win == nullptr || dst == nullptr) return E::ACL_ERROR_INTERNAL_ERROR;
+        int cur = 0;
+        if (hipGetDevice(&cur) == hipSuccess && cur != p->device) SIP_HIP_CHECK(hipSetDevice(p->device));
+        sip::plan_execute_istft(*p, src, static_cast<const float *>(win), dst);
+        return E::ACL_SUCCESS;
+    });
 }
 
 int32_t asdFftSynchronize(void *handle)

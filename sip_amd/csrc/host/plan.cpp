@@ -727,6 +727,24 @@ void plan_build(Plan &p)
     OUT.kind = BUF_OUT;
     const int64_t B = p.batch;
 
+    if (p.istft.active) {
+        // spectra X[c][k][t] (element k of frame (c, t) at c*n*T + k*T + t): inverse DFTs straight
+        // from that layout (column tiles over t, no transpose) into frames F[c][t][j], then the
+        // windowed overlap-add (reference: transpose + C2C + unfold-backward + envelope division,
+        // core/fft/istft.cpp:91-104, core/fft/fftcore/fft_core_istft_any.cpp:292-380)
+        const Plan::Istft &q = p.istft;
+        BufRef fr = b.new_ws((size_t)q.channel * q.frames * q.nfft * 8);
+        Lay in{IN, q.frames, {{q.frames, 1}, {q.channel, q.nfft * q.frames}}};
+        Lay fl{fr, 1, {{q.frames, q.nfft}, {q.channel, q.frames * q.nfft}}};
+        b.axis(q.nfft, +1, in, fl, false);
+        b.add_aux(AUX_ISTFT_OLA, Lay{fr, 1, {{q.channel, q.frames * q.nfft}}}, Lay{OUT, 1, {{q.channel, q.out_len}}},
+                  q.nfft, q.out_len, TableRef(), 1.f, "istft overlap-add");
+        AuxArgs &x = p.stages.back().xa;
+        x.hop = q.hop;
+        x.frames = q.frames;
+        x.start = q.nfft / 2;
+        return;
+    }
     if (nd == 1) {
         const int64_t N = p.sizes[0];
         if (p.fftType == kTypeC2C || p.fftType == kTypeC2CSep) {
@@ -945,6 +963,13 @@ static void *resolve(const Plan &p, const BufRef &b, const void *in, void *out)
 }
 
 void plan_execute(Plan &p, const void *in, void *out) { plan_execute_sep(p, in, nullptr, out, nullptr); }
+
+void plan_execute_istft(Plan &p, const void *in, const float *window, void *out)
+{
+    for (Stage &s : p.stages)
+        if (s.kind == StageKind::Aux && s.xa.mode == AUX_ISTFT_OLA) s.xa.win = window;
+    plan_execute(p, in, out);
+}
 
 void plan_execute_sep(Plan &p, const void *in, const void *in_im, void *out, void *out_im)
 {

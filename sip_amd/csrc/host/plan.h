@@ -111,6 +111,12 @@ struct Plan {
     std::vector<const int64_t *> external_tables;  // caller-owned device row tables (serving layer)
     Tune tune;
     int precision = 0;  // PrecisionMode of the radix-16 stages of C2C launches
+    // ISTFT plan (asdFftIstftMakePlan): inverse C2C of the (channel, n_fft, frames) spectra into a
+    // frame workspace, then the windowed overlap-add into (channel, out_len)
+    struct Istft {
+        bool active = false;
+        int64_t channel = 0, nfft = 0, frames = 0, hop = 0, win_len = 0, out_len = 0;
+    } istft;
 
     int sign() const { return direction == kDirInverse ? 1 : -1; }
     ~Plan();
@@ -136,6 +142,8 @@ void plan_materialize(Plan &p);
 void plan_execute(Plan &p, const void *in, void *out);
 // Split-plane (C2C_SEP) execution: user buffers are real / imaginary float planes.
 void plan_execute_sep(Plan &p, const void *in, const void *in_im, void *out, void *out_im);
+// ISTFT execution: `window` (n_fft floats) feeds the overlap-add stage.
+void plan_execute_istft(Plan &p, const void *in, const float *window, void *out);
 // JSON description of the stages.
 std::string plan_describe(const Plan &p);
 
@@ -153,5 +161,7 @@ hipError_t aux_launch(const AuxArgs &a, hipStream_t stream);
 // core/fft/fft_api.cpp:169-245, 383-697 and core/fft/fftcore/select_core.cpp:155-200).
 std::vector<std::string> reference_steps(int fftType, int direction, const std::vector<int64_t> &sizes,
                                          int64_t batch, int dimType);
+// The reference's steps of an ISTFT plan (core/fft/istft.cpp:91-104).
+std::vector<std::string> reference_istft_steps(int64_t channel, int64_t nfft, int64_t frames, int64_t hop);
 
 }  // namespace sip

@@ -231,6 +231,18 @@ bool support_fusing(int64_t x, int64_t y, int rx, int ry, int type)
 
 }  // namespace
 
+std::vector<std::string> reference_istft_steps(int64_t channel, int64_t nfft, int64_t frames, int64_t hop)
+{
+    // core/fft/istft.cpp:91-104: transpose (channel, n_fft, frames) -> (channel, frames, n_fft),
+    // the inverse C2C of n_fft over channel * frames rows, then the ISTFT overlap-add core
+    std::vector<std::string> s;
+    s.push_back(transpose_step(1, 2, {channel, nfft, frames}));
+    s.push_back(core_step(kTypeC2C, nfft, 1, channel * frames, false));
+    s.push_back("IstftAny(n_fft=" + std::to_string(nfft) + ",hop=" + std::to_string(hop) +
+                ",frames=" + std::to_string(frames) + ",channel=" + std::to_string(channel) + ")");
+    return s;
+}
+
 std::vector<std::string> reference_steps(int type, int direction, const std::vector<int64_t> &sizes, int64_t batch,
                                          int dimType)
 {

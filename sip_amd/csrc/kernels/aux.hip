@@ -98,6 +98,29 @@ __global__ __launch_bounds__(256) void aux_kernel(const AuxArgs a)
             case AUX_BLUE_MUL:
                 cout[ob + k * a.rout.es] = cmul(cin[ib + k * a.rin.es], a.tab0[k]);
                 break;
+            case AUX_ISTFT_OLA: {
+                // Output sample k of channel r is m = k + start of the overlap-add span; the frames
+                // t with t*hop <= m < t*hop + n cover it. y = sum_t frame_t[m - t hop] w[m - t hop],
+                // env = sum_t w[m - t hop]^2, out = y / (n env) (the 1/n of the inverse DFT).
+                const int64_t n = a.n, hop = a.hop;
+                const int64_t m = k + a.start;
+                int64_t t_lo = m >= n ? (m - n) / hop + 1 : 0;
+                int64_t t_hi = m / hop;
+                if (t_hi > a.frames - 1) t_hi = a.frames - 1;
+                float2 acc = make_float2(0.f, 0.f);
+                float env = 0.f;
+                for (int64_t t = t_lo; t <= t_hi; t++) {
+                    const int64_t j = m - t * hop;
+                    const float w = a.win[j];
+                    const float2 f = cin[ib + t * n + j];
+                    acc.x = fmaf(f.x, w, acc.x);
+                    acc.y = fmaf(f.y, w, acc.y);
+                    env = fmaf(w, w, env);
+                }
+                const float d = (float)n * env;
+                cout[ob + k * a.rout.es] = make_float2(acc.x / d, acc.y / d);
+                break;
+            }
             case AUX_BLUE_POST:
                 cout_at(a, ob + k * a.rout.es, cscale(cmul(cin[ib + k * a.rin.es], a.tab0[k]), a.scale));
                 break;

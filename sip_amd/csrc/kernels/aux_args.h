@@ -18,6 +18,7 @@ enum AuxMode : int32_t {
     AUX_BLUE_PRE = 6,      // out[m] = m < n ? in[m] * tab0[m] : 0,     m < out_len (= M)
     AUX_BLUE_MUL = 7,      // out[m] = in[m] * tab0[m],                 m < out_len
     AUX_BLUE_POST = 8,     // out[k] = in[k] * tab0[k] * scale,         k < out_len (= n)
+    AUX_ISTFT_OLA = 9,     // windowed overlap-add of n-point frames (hop, frames, start; see aux.hip)
 };
 
 struct AuxRows {
@@ -33,6 +34,8 @@ struct AuxArgs {
     const float *in_im;   // split-plane input (C2C modes): real parts at (float *)in
     float *out_im;        // split-plane output: real parts at (float *)out
     int32_t in_sep, out_sep;
+    const float *win;     // ISTFT window (n floats)
+    int64_t hop, frames, start;
     const float2 *tab0;
     int64_t rows;
     int64_t n;          // logical transform length

@@ -80,9 +80,36 @@ def test_exec_on_uninitialised_plan_and_bad_tensors():
     assert L.asdFftExecC2C(h.h, None, None) == INV
     # separated exec on a plan that is not C2C_SEP: invalid parameter; ISTFT answers "not supported"
     assert L.asdFftExecC2CSeparated(h.h, one.ptr, one.ptr, one.ptr, one.ptr) == INV
-    assert L.asdFftIstftMakePlan(h.h, one.ptr, 16, 4, 16, True, False, False, 0, True) == UNSUP
-    assert L.asdFftExecIstft(h.h, one.ptr, None, one.ptr) == UNSUP
+    # ISTFT: a 1-D input is not a (channel, n_fft, frames) spectrum; exec without an ISTFT plan
+    assert L.asdFftIstftMakePlan(h.h, one.ptr, 16, 4, 16, True, False, False, 0, True) == INV
+    assert L.asdFftExecIstft(h.h, one.ptr, None, one.ptr) == INV
     one.close()
+    h.destroy()
+
+
+def test_istft_make_plan_validation():
+    """IstftParamsCheck rules (core/fft/istft.cpp:305-410), all answered before any device work."""
+    L = S.lib()
+    h = S.FftHandle()
+
+    def mk(shape, n_fft, hop, win, handle=None):
+        t = S.AclTensor(0x1000, shape, S.ACL_COMPLEX64)
+        try:
+            return L.asdFftIstftMakePlan(h.h if handle is None else handle, t.ptr, n_fft, hop, win, True, False,
+                                         False, 0, True)
+        finally:
+            t.close()
+
+    assert mk([2, 16, 1], 16, 4, 16) == INV            # frames must exceed 1
+    assert mk([2, 16, 5], 32, 4, 32) == INV            # n_fft must equal the frequency dim
+    assert mk([2, 16, 5], 16, 0, 16) == INV            # 0 < hop
+    assert mk([2, 16, 5], 16, 17, 16) == INV           # hop <= win_length
+    assert mk([2, 16, 5], 16, 4, 12) == INV            # win_length == n_fft
+    assert mk([1, 1500, 3], 1500, 4, 1500) == INV      # n_fft < 1500
+    assert mk([2, 16, 5], 16, 4, 16, handle=ctypes.c_void_p(0x12345)) == INV  # unknown handle
+    real = S.AclTensor(0x1000, [2, 16, 5], S.ACL_FLOAT)
+    assert L.asdFftIstftMakePlan(h.h, real.ptr, 16, 4, 16, True, False, False, 0, True) == INV  # complex64 only
+    real.close()
     h.destroy()
 
 

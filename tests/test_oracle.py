@@ -133,3 +133,52 @@ def test_planted_sign_mistake_is_caught():
     x = oracle.uniform_complex((1, 64), 5).astype(np.complex128)
     wrong = oracle.c2c(x, oracle.INVERSE)
     assert np.abs(wrong - np.fft.fft(x)).max() > 1e-3
+
+
+# ---------------------------------------------------------------- ISTFT restatement
+
+def _torch_cpu():
+    torch = pytest.importorskip("torch")
+    return torch
+
+
+@pytest.mark.parametrize("n_fft,hop,frames,ch", [(16, 4, 9, 2), (64, 16, 12, 3), (400, 100, 7, 1), (101, 37, 5, 2)])
+def test_istft_matches_torch_istft(n_fft, hop, frames, ch):
+    """torch.istft is an independent implementation of the same definition (center=True,
+    normalized=False, onesided=False, return_complex=True, length=None)."""
+    torch = _torch_cpu()
+    rng = np.random.default_rng(n_fft * 7 + hop)
+    x = rng.standard_normal((ch, n_fft, frames)) + 1j * rng.standard_normal((ch, n_fft, frames))
+    w = np.hanning(n_fft + 2)[1:-1] + 0.05  # strictly positive, so every sample has envelope
+    ref = torch.istft(torch.from_numpy(x), n_fft=n_fft, hop_length=hop, win_length=n_fft,
+                      window=torch.from_numpy(w), center=True, normalized=False, onesided=False,
+                      return_complex=True).numpy()
+    got = oracle.istft(x, n_fft, hop, w)
+    assert got.shape == ref.shape == (ch, oracle.istft_out_len(n_fft, hop, frames))
+    assert np.max(np.abs(got - ref)) <= 1e-9 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_istft_inverts_stft():
+    """STFT (torch.stft, same framing) followed by the oracle ISTFT returns the signal."""
+    torch = _torch_cpu()
+    n_fft, hop = 256, 64
+    rng = np.random.default_rng(5)
+    sig = rng.standard_normal((2, 4000)) + 1j * rng.standard_normal((2, 4000))
+    w = np.hanning(n_fft + 1)[:-1]
+    spec = torch.stft(torch.from_numpy(sig), n_fft=n_fft, hop_length=hop, win_length=n_fft,
+                      window=torch.from_numpy(w), center=True, onesided=False, return_complex=True).numpy()
+    back = oracle.istft(spec, n_fft, hop, w)
+    m = back.shape[1]
+    assert m == oracle.istft_out_len(n_fft, hop, spec.shape[2])
+    assert np.max(np.abs(back - sig[:, :m])) < 1e-10
+
+
+def test_istft_closed_form_single_tone():
+    """One frame pair with a rectangular window: the spectrum of a constant frame (n_fft at bin 0)
+    gives back the constant wherever the frames overlap."""
+    n_fft, hop, frames = 8, 4, 3
+    x = np.zeros((1, n_fft, frames), dtype=np.complex128)
+    x[0, 0, :] = n_fft * (2.0 - 1.0j)      # every frame is the constant 2 - i
+    y = oracle.istft(x, n_fft, hop, np.ones(n_fft))
+    assert y.shape == (1, oracle.istft_out_len(n_fft, hop, frames))
+    assert np.allclose(y, 2.0 - 1.0j, atol=1e-13)
