@@ -539,21 +539,37 @@ __device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds,
             }
         }
     } else {
+        // generic complex tile (e.g. the L+1 bins of a C2R row): in_len <= L + 1 and the block is
+        // F * T threads, so a thread moves at most NG elements; all loads are issued first
+        constexpr int NG = (L + 1 + T - 1) / T;
         const float2 *src = static_cast<const float2 *>(a.in);
         const int total = F * in_len;
-        for (int i = tid; i < total; i += nthreads) {
-            int fl, j;
-            if (a.tin.unit_f) {
-                j = (int)fdiv((uint32_t)i, a.div_F);
-                fl = i - j * F;
-            } else {
-                fl = (int)fdiv((uint32_t)i, a.div_in_len);
-                j = i - fl * in_len;
+        float2 v[NG];
+        int dst[NG];
+#pragma unroll
+        for (int u = 0; u < NG; u++) {
+            const int i = tid + u * nthreads;
+            dst[u] = -1;
+            if (i < total) {
+                int fl, j;
+                if (a.tin.unit_f) {
+                    j = (int)fdiv((uint32_t)i, a.div_F);
+                    fl = i - j * F;
+                } else {
+                    fl = (int)fdiv((uint32_t)i, a.div_in_len);
+                    j = i - fl * in_len;
+                }
+                if (fl < nf) {
+                    v[u] = src[in_base[fl] + (int64_t)j * a.tin.es];
+                    dst[u] = fl * LP + lds_pad(j);
+                }
             }
-            if (fl < nf) {
-                const float2 v = src[in_base[fl] + (int64_t)j * a.tin.es];
-                re[fl * LP + lds_pad(j)] = v.x;
-                im[fl * LP + lds_pad(j)] = v.y;
+        }
+#pragma unroll
+        for (int u = 0; u < NG; u++) {
+            if (dst[u] >= 0) {
+                re[dst[u]] = v[u].x;
+                im[dst[u]] = v[u].y;
             }
         }
     }
@@ -562,11 +578,27 @@ __device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds,
     // ---------------------------------------------------------------- pre-process (C2R)
     if (a.io_mode == IO_C2R_EVEN) {
         // P[k] = (X[k] + conj X[L-k]) + i w^k (X[k] - conj X[L-k]); imag of X[0] and X[L] dropped.
+        // The block is F * T threads: a thread handles at most NP pairs; the root loads of all of
+        // them are issued before the first LDS access.
         constexpr int NPAIR = L / 2 + 1;
-        for (int i = tid; i < F * NPAIR; i += nthreads) {
+        constexpr int NP = (NPAIR + T - 1) / T;
+        float2 w1[NP], w2[NP];
+#pragma unroll
+        for (int u = 0; u < NP; u++) {
+            const int i = tid + u * nthreads;
             const int fl = i / NPAIR;
             const int k = i - fl * NPAIR;
-            if (fl >= nf) continue;
+            if (i < F * NPAIR && fl < nf) {
+                w1[u] = a.rtw[k];
+                w2[u] = a.rtw[L - k];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < NP; u++) {
+            const int i = tid + u * nthreads;
+            const int fl = i / NPAIR;
+            const int k = i - fl * NPAIR;
+            if (i >= F * NPAIR || fl >= nf) continue;
             float *pr = re + fl * LP, *pi = im + fl * LP;
             const int k2 = L - k;
             float2 xk = make_float2(pr[lds_pad(k)], pi[lds_pad(k)]);
@@ -578,14 +610,14 @@ __device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds,
             {
                 const float2 c = cconj(xm);
                 const float2 s = cadd(xk, c), d = csub(xk, c);
-                const float2 p = cadd(s, mul_si<1>(cmul(a.rtw[k], d)));
+                const float2 p = cadd(s, mul_si<1>(cmul(w1[u], d)));
                 pr[lds_pad(k)] = p.x;
                 pi[lds_pad(k)] = p.y;
             }
             if (k != 0 && k2 != k) {
                 const float2 c = cconj(xk);
                 const float2 s = cadd(xm, c), d = csub(xm, c);
-                const float2 p = cadd(s, mul_si<1>(cmul(a.rtw[k2], d)));
+                const float2 p = cadd(s, mul_si<1>(cmul(w2[u], d)));
                 pr[lds_pad(k2)] = p.x;
                 pi[lds_pad(k2)] = p.y;
             }
