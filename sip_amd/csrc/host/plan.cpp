@@ -525,6 +525,33 @@ struct Builder {
         p.stages.push_back(st);
     }
 
+    // Fused 2D tiles: both axes of `slices` contiguous X x Y slices in one launch (in -> out).
+    void add_tile2d(const Stockham2dInstance *t, int sign, const BufRef &in, const BufRef &out, int64_t slices,
+                    const std::string &note)
+    {
+        if (slices >= (int64_t(1) << 31))
+            throw SipError(AsdSip::ACL_ERROR_API_NOT_SUPPORT, "more than 2^31 slices in one launch");
+        Stage st;
+        st.kind = StageKind::Stockham;
+        st.in = in;
+        st.out = out;
+        st.inst = find_instance(t->Y);
+        st.inst2d = t;
+        st.sign = sign;
+        std::memset(&st.sa, 0, sizeof(st.sa));
+        st.sa.nfft = slices;
+        st.sa.io_mode = IO_C2C;
+        st.sa.F = 1;
+        st.sa.in_len = st.sa.out_len = t->X * t->Y;
+        st.grid = dim3((unsigned)slices);
+        st.block = dim3((unsigned)t->threads);
+        st.lds = (size_t)t->lds_bytes;
+        st.tw = pass_twiddles(find_instance(t->Y), sign);
+        st.tw2 = pass_twiddles(find_instance(t->X), sign);
+        st.note = note;
+        p.stages.push_back(st);
+    }
+
     // Two-leaf split L = L1 * L2 for a four-step pair. Stage B stores transposed (a column tile),
     // so both factors must be column leaves; the most balanced pair is taken.
     static bool leaf_split(int64_t L, int64_t &L1, int64_t &L2)
@@ -771,6 +798,24 @@ void plan_build(Plan &p)
     // 2D / 3D: array [B][d0][d1]([d2]), last axis fastest.
     std::vector<int64_t> d = p.sizes;
     if (p.fftType == kTypeC2C || p.fftType == kTypeC2CSep) {
+        // the last two axes of 32..128 each: one fused 2D tile launch (in -> out), then the first
+        // axis of a 3D transform in place on out
+        const Stockham2dInstance *t2 = (!b.sep && p.precision == PREC_FP32 && env_flag("SIP_FFT_FUSED2D", true))
+                                           ? find_2d_instance(d[nd - 2], d[nd - 1]) : nullptr;
+        if (t2) {
+            int64_t slices = B;
+            for (int i = 0; i < nd - 2; i++) slices *= d[i];
+            b.add_tile2d(t2, sign, IN, OUT, slices,
+                         "fused 2D " + std::to_string(d[nd - 2]) + "x" + std::to_string(d[nd - 1]));
+            for (int ax = nd - 3; ax >= 0; ax--) {
+                int64_t inner = 1, outer = B;
+                for (int i = ax + 1; i < nd; i++) inner *= d[i];
+                for (int i = 0; i < ax; i++) outer *= d[i];
+                Lay l{OUT, inner, {{inner, 1}, {outer, d[ax] * inner}}};
+                b.axis(d[ax], sign, l, l, true);
+            }
+            return;
+        }
         // last axis: rows in -> out, then the other axes in place on out
         int64_t rows = B;
         for (int i = 0; i < nd - 1; i++) rows *= d[i];
@@ -891,6 +936,7 @@ Plan::~Plan()
 
 static const void *stage_kernel(const Stage &s)
 {
+    if (s.inst2d) return s.sign < 0 ? s.inst2d->fwd : s.inst2d->inv;
     if (s.sa.split_lds) return s.sign < 0 ? s.inst->rows_fwd : s.inst->rows_inv;
     return s.sign < 0 ? s.inst->fwd : s.inst->inv;
 }
@@ -935,6 +981,10 @@ void plan_materialize(Plan &p)
                 s.tw4hi.dev = coeff_get(s.tw4hi);
             }
             s.sa.tw = static_cast<const float2 *>(s.tw.dev->ptr);
+            if (s.tw2.used()) {
+                s.tw2.dev = coeff_get(s.tw2);
+                s.sa.tw2 = static_cast<const float2 *>(s.tw2.dev->ptr);
+            }
             if (s.mfw.used()) {
                 s.mfw.dev = coeff_get(s.mfw);
                 s.sa.mfma_w = static_cast<const unsigned short *>(s.mfw.dev->ptr);
@@ -1037,6 +1087,7 @@ std::string plan_describe(const Plan &p)
                << ",\"split_lds\":" << s.sa.split_lds << ",\"col_direct\":" << s.sa.col_direct
                << ",\"in_sep\":" << s.sa.in_sep << ",\"out_sep\":" << s.sa.out_sep << ",\"mfma\":" << (s.mfw.used() ? 1 : 0)
                << ",\"xcd_remap\":" << s.sa.xcd_remap;
+            if (s.inst2d) os << ",\"tile2d\":[" << s.inst2d->X << "," << s.inst2d->Y << "]";
         } else {
             os << ",\"mode\":" << s.xa.mode << ",\"rows\":" << s.xa.rows << ",\"n\":" << s.xa.n
                << ",\"out_len\":" << s.xa.out_len;

@@ -77,6 +77,9 @@ struct Stage {
     dim3 grid, block;
     size_t lds = 0;
     TableRef tw, rtw, tw4lo, tw4hi, mfw;
+    // fused 2D tile stage (both axes of X x Y slices in one launch); tw2 = column twiddles
+    const Stockham2dInstance *inst2d = nullptr;
+    TableRef tw2;
     int in_tab = -1, out_tab = -1;  // row tables (Plan::row_tables) gathered by the in / out tiles
     // Aux
     AuxArgs xa{};
@@ -151,6 +154,8 @@ std::string plan_describe(const Plan &p);
 const StockhamInstance *find_instance(int64_t L);
 // Split-precision MFMA instances (mode = PrecisionMode) and their DFT-matrix operand table.
 const StockhamInstance *find_mfma_instance(int64_t L, int mode);
+// Fused 2D tile instance for X x Y slices (X, Y in 32, 64, 128), or null.
+const Stockham2dInstance *find_2d_instance(int64_t X, int64_t Y);
 TableRef mfma_dft_table(int mode, int sign);
 int64_t instance_count();
 

@@ -34,4 +34,13 @@ const StockhamInstance *find_instance(int64_t L)
 
 int64_t instance_count() { return (int64_t)instances().size(); }
 
+extern const Stockham2dInstance g_stockham2d[9];
+
+const Stockham2dInstance *find_2d_instance(int64_t X, int64_t Y)
+{
+    for (const Stockham2dInstance &t : g_stockham2d)
+        if (t.X == X && t.Y == Y) return &t;
+    return nullptr;
+}
+
 }  // namespace sip

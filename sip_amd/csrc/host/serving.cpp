@@ -479,7 +479,7 @@ private:
         auto &held = resident_[std::make_pair(type, n)];
         if (!held.empty()) return;
         for (const Stage &st : plan.stages)
-            for (const TableRef *t : {&st.tw, &st.rtw, &st.tw4lo, &st.tw4hi, &st.mfw, &st.tab0})
+            for (const TableRef *t : {&st.tw, &st.rtw, &st.tw4lo, &st.tw4hi, &st.mfw, &st.tab0, &st.tw2})
                 if (t->dev) held.push_back(t->dev);
         if (held.empty()) held.push_back(nullptr);  // marks the shape as seen
     }

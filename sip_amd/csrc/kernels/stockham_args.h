@@ -50,6 +50,7 @@ struct StockhamArgs {
     void *out;
     const float2 *tw;      // pass twiddles (FP64-generated, rounded to FP32), layout fixed by the radix sequence
     const float2 *rtw;     // real-transform roots w^k = e^{s 2 pi i k/(2L)}, k = 0..L
+    const float2 *tw2;     // fused 2D tiles: pass twiddles of the column (X) transforms
     const float2 *tw4_lo;  // four-step twiddles W^m = lo[m & mask] * hi[m >> shift], W = e^{s 2 pi i/Ntot}
     const float2 *tw4_hi;
     const unsigned short *mfma_w;  // split DFT-matrix fragments of the MFMA radix-16 stages
@@ -101,6 +102,16 @@ struct StockhamInstance {
     const void *inv;
     const void *rows_fwd;    // single-plane contiguous-row kernels (T >= 16 only, else null)
     const void *rows_inv;
+};
+
+// Registry entry of a fused 2D tile instance (stockham2d.h). The radix sequences, and so the
+// twiddle tables, are those of the 1D leaf instances of lengths X and Y.
+struct Stockham2dInstance {
+    int32_t X, TX, Y, TY;
+    int32_t threads;
+    int32_t lds_bytes;
+    const void *fwd;
+    const void *inv;
 };
 
 }  // namespace sip

@@ -107,3 +107,27 @@ def test_inplace_2d(torch_cuda):
     rel, _ = report("in-place 2d", xd.cpu().numpy(), oracle.c2c_nd(x, 2))
     h.destroy()
     assert rel <= TOL
+
+
+@pytest.mark.parametrize("x", [32, 64, 128])
+@pytest.mark.parametrize("y", [32, 64, 128])
+def test_fused_2d_tiles(torch_cuda, x, y):
+    """Every fused 2D tile instance (both axes of an X x Y slice in one launch), both directions,
+    and a 3D transform whose last two axes take the fused path."""
+    import sip_amd as S
+    from fftcase import make
+    b = 5
+    a = oracle.uniform_complex((b, x, y), 0x5EED0300 + x * 1000 + y)
+    for direction in (FWD, INV):
+        h = make([x, y], S.ASCEND_FFT_C2C, direction, b)
+        try:
+            assert h.describe()["stages"][0].get("tile2d") == [x, y]
+        finally:
+            h.destroy()
+        got = c2c(a, [x, y], direction)
+        rel, _ = report(f"fused 2d {x}x{y} b={b} dir={direction:#x}", got, oracle.c2c_nd(a, 2, sign_of(direction)))
+        assert rel <= 2e-6
+    a3 = oracle.uniform_complex((2, 12, x, y), 0x5EED0400 + x + y)
+    got = c2c(a3, [12, x, y], FWD)
+    rel, _ = report(f"3d 12x{x}x{y} (fused last two axes)", got, oracle.c2c_nd(a3, 3))
+    assert rel <= 2e-6

@@ -178,7 +178,11 @@ def test_plan_layouts():
     d = dry([1024, 1024], S.ASCEND_FFT_C2C, 8)
     assert d["workspace"] == 0 and len(d["stages"]) == 2 and d["stages"][1]["unit_f_in"] == 1
     d = dry([128, 128, 128], S.ASCEND_FFT_C2C, 4)
-    assert d["workspace"] == 0 and len(d["stages"]) == 3
+    # the last two axes in one fused 2D tile launch, then the first axis in place
+    assert d["workspace"] == 0 and len(d["stages"]) == 2 and d["stages"][0]["tile2d"] == [128, 128]
+    assert d["stages"][1]["in"] == d["stages"][1]["out"] == "out"
+    d = dry([64, 32], S.ASCEND_FFT_C2C, 3)
+    assert len(d["stages"]) == 1 and d["stages"][0]["tile2d"] == [64, 32] and d["stages"][0]["grid"] == 3
     d = dry([8192], S.ASCEND_FFT_R2C, 4096)
     assert len(d["stages"]) == 1 and d["stages"][0]["io"] == 1 and d["stages"][0]["L"] == 4096
     d = dry([100, 200], S.ASCEND_FFT_C2R, 2)
