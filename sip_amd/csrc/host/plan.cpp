@@ -53,7 +53,7 @@ static float2 root_f(int64_t m, int64_t n, int sign)
 static TableRef pass_twiddles(const StockhamInstance *in, int sign)
 {
     TableRef t;
-    t.key = "twk:" + std::to_string(in->L);
+    t.key = "twe:" + std::to_string(in->L);
     for (int p = 0; p < in->npass; p++) t.key += "," + std::to_string(in->radix[p]);
     t.key += ":" + std::to_string(sign);
     t.gen = [in, sign]() {
@@ -78,6 +78,8 @@ static TableRef pass_twiddles(const StockhamInstance *in, int sign)
             } else {
                 for (int64_t r = 1; r < R; r++)
                     for (int64_t k = 0; k < ns; k++) v.push_back(root_f(k * r * step, L, sign));
+                if ((R - 1) % 2)  // pad to an even row count (16-byte aligned pass tables)
+                    for (int64_t k = 0; k < ns; k++) v.push_back(float2{0.f, 0.f});
             }
             ns *= R;
         }

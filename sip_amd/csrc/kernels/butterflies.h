@@ -3,7 +3,9 @@
 // dft<R, S>(v) overwrites v[0..R-1] with V[q] = sum_r v[r] e^{S 2 pi i r q / R}; S = -1 forward,
 // S = +1 inverse. Radix 4/8/16 are split-radix style decompositions into 2- and 4-point DFTs
 // with the internal roots written as FP32 roundings of their FP64 values; 3 and 5 use the
-// conjugate-pair (Winograd-style) form; 7, 11 and 13 use the same pairing with constant tables.
+// conjugate-pair (Winograd-style) form; 7, 11 and 13 use the same pairing with constant tables;
+// 6, 9, 10, 12, 15, 20 and 25 are two-factor Cooley-Tukey products of those with compile-time
+// roots (double precision, rounded once).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -215,6 +217,109 @@ __device__ __forceinline__ void dft16(float2 *v)
     }
 }
 
+// cos / sin (2 pi m / n), evaluated at compile time in double (Taylor series on [0, pi/4] after
+// quarter-turn and octant reduction; quarter points exact) and rounded once to FP32.
+constexpr double kCePi = 3.14159265358979323846;
+constexpr double ce_sin_poly(double x)
+{
+    double x2 = x * x, term = x, sum = x;
+    for (int k = 1; k < 14; k++) {
+        term *= -x2 / ((2.0 * k) * (2.0 * k + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double ce_cos_poly(double x)
+{
+    double x2 = x * x, term = 1.0, sum = 1.0;
+    for (int k = 1; k < 14; k++) {
+        term *= -x2 / ((2.0 * k - 1.0) * (2.0 * k));
+        sum += term;
+    }
+    return sum;
+}
+constexpr void ce_cossin2pi(int m, int n, double &c, double &s)
+{
+    m %= n;
+    const int q = (4 * m) / n;          // quarter turn
+    const int rem = 4 * m - q * n;      // remainder in units of 1/(4n) turns
+    double cc, ss;
+    if (rem == 0) {
+        cc = 1.0;
+        ss = 0.0;
+    } else {
+        const double x = kCePi / 2.0 * (double)rem / (double)n;  // in (0, pi/2)
+        if (2 * rem <= n) {
+            cc = ce_cos_poly(x);
+            ss = ce_sin_poly(x);
+        } else {
+            const double y = kCePi / 2.0 * (double)(n - rem) / (double)n;
+            cc = ce_sin_poly(y);
+            ss = ce_cos_poly(y);
+        }
+    }
+    for (int k = 0; k < q; k++) {  // rotate by q quarter turns
+        const double t = cc;
+        cc = -ss;
+        ss = t;
+    }
+    c = cc;
+    s = ss;
+}
+
+template <int R>
+struct RootTable {
+    float c[R], s[R];
+    constexpr RootTable() : c(), s()
+    {
+        for (int m = 0; m < R; m++) {
+            double cc = 0, ss = 0;
+            ce_cossin2pi(m, R, cc, ss);
+            c[m] = (float)cc;
+            s[m] = (float)ss;
+        }
+    }
+};
+template <int R>
+constexpr RootTable<R> kRoots = RootTable<R>();
+
+template <int R, int S>
+__device__ __forceinline__ void dft(float2 *v);
+
+// Composite radix R = P * Q in registers (Cooley-Tukey with the internal roots as constants):
+// y[n2][k1] = DFT_P over n1 of v[Q n1 + n2], y[n2][k1] *= W_R^{n2 k1}, V[k1 + P k2] = DFT_Q over n2.
+template <int P, int Q, int S>
+__device__ __forceinline__ void dft_pq(float2 *v)
+{
+    constexpr int R = P * Q;
+    float2 y[Q][P];
+#pragma unroll
+    for (int n2 = 0; n2 < Q; n2++) {
+        float2 t[P];
+#pragma unroll
+        for (int n1 = 0; n1 < P; n1++) t[n1] = v[Q * n1 + n2];
+        dft<P, S>(t);
+#pragma unroll
+        for (int k1 = 0; k1 < P; k1++) y[n2][k1] = t[k1];
+    }
+#pragma unroll
+    for (int n2 = 1; n2 < Q; n2++)
+#pragma unroll
+        for (int k1 = 1; k1 < P; k1++) {
+            const int m = (n2 * k1) % R;
+            y[n2][k1] = cmul(y[n2][k1], make_float2(kRoots<R>.c[m], S * kRoots<R>.s[m]));
+        }
+#pragma unroll
+    for (int k1 = 0; k1 < P; k1++) {
+        float2 t[Q];
+#pragma unroll
+        for (int n2 = 0; n2 < Q; n2++) t[n2] = y[n2][k1];
+        dft<Q, S>(t);
+#pragma unroll
+        for (int k2 = 0; k2 < Q; k2++) v[k1 + P * k2] = t[k2];
+    }
+}
+
 template <int R, int S>
 __device__ __forceinline__ void dft(float2 *v)
 {
@@ -230,6 +335,20 @@ __device__ __forceinline__ void dft(float2 *v)
         dft8<S>(v);
     } else if constexpr (R == 16) {
         dft16<S>(v);
+    } else if constexpr (R == 6) {
+        dft_pq<2, 3, S>(v);
+    } else if constexpr (R == 9) {
+        dft_pq<3, 3, S>(v);
+    } else if constexpr (R == 10) {
+        dft_pq<2, 5, S>(v);
+    } else if constexpr (R == 12) {
+        dft_pq<4, 3, S>(v);
+    } else if constexpr (R == 15) {
+        dft_pq<3, 5, S>(v);
+    } else if constexpr (R == 20) {
+        dft_pq<4, 5, S>(v);
+    } else if constexpr (R == 25) {
+        dft_pq<5, 5, S>(v);
     } else {
         dft_prime<R, S>(v);
     }

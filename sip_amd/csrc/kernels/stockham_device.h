@@ -92,8 +92,8 @@ __device__ __forceinline__ int lds_idx_write(int hi, int k, int r)
 // (the other r are products of those, at most three multiplies deep), k-major and padded to an
 // even count, so a radix-16 or radix-8 pass loads its twiddles with two 16-byte loads: layout
 // [k][4] (radix 8 and 16), [k][2] (radix 4; radix 2 uses the first slot). Other radices store
-// every r = 1 .. R-1 row-major, [r-1][k]. Every pass has an even number of rows, so every pass
-// table starts 16-byte aligned.
+// every r = 1 .. R-1 row-major, [r-1][k], padded to an even row count. Every pass has an even
+// number of rows, so every pass table starts 16-byte aligned.
 template <int R>
 constexpr int tw_rows()
 {
@@ -103,7 +103,7 @@ constexpr int tw_rows()
         return 4;
     } else {
         static_assert((R & (R - 1)) != 0, "power-of-two radices above 16 have no twiddle layout");
-        return R - 1;
+        return (R - 1 + 1) & ~1;
     }
 }
 
