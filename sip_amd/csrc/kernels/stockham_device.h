@@ -688,21 +688,34 @@ __device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds,
             }
         }
     } else if (a.io_mode == IO_R2C_EVEN) {
-        // X[k] = Z[k] (1 - i w^k)/2 + conj Z[L-k] (1 + i w^k)/2, k = 0..L
+        // X[k] = Z[k] (1 - i w^k)/2 + conj Z[L-k] (1 + i w^k)/2, k = 0..L. The bins k and L - k
+        // use the same two LDS values: one thread computes both (pairs k = 0..L/2).
         float2 *dst = static_cast<float2 *>(a.out);
-        for (int i = tid; i < F * out_len; i += nthreads) {
-            const int fl = (int)fdiv((uint32_t)i, a.div_out_len);
-            const int k = i - fl * out_len;
+        constexpr int NPAIR = L / 2 + 1;
+        for (int i = tid; i < F * NPAIR; i += nthreads) {
+            const int fl = i / NPAIR;
+            const int k = i - fl * NPAIR;
             if (fl >= nf) continue;
             const float *pr = re + fl * LP, *pi = im + fl * LP;
-            const int ka = (k == L) ? 0 : k;
-            const int kb = (k == 0) ? 0 : L - k;
-            const float2 z = make_float2(pr[lds_pad(ka)], pi[lds_pad(ka)]);
-            const float2 zc = make_float2(pr[lds_pad(kb)], -pi[lds_pad(kb)]);
-            const float2 iw = mul_si<1>(a.rtw[k]);
-            const float2 A = make_float2(0.5f * (1.f - iw.x), -0.5f * iw.y);
-            const float2 B = make_float2(0.5f * (1.f + iw.x), 0.5f * iw.y);
-            dst[out_base[fl] + (int64_t)k * a.tout.es] = cadd(cmul(z, A), cmul(zc, B));
+            const int k2 = L - k;
+            const int ka = k, kb = (k == 0) ? 0 : k2;
+            const float2 zk = make_float2(pr[lds_pad(ka)], pi[lds_pad(ka)]);
+            const float2 zm = make_float2(pr[lds_pad(kb)], pi[lds_pad(kb)]);
+            float2 *o = dst + out_base[fl];
+            const int64_t es = a.tout.es;
+            {
+                const float2 iw = mul_si<1>(a.rtw[k]);
+                const float2 A = make_float2(0.5f * (1.f - iw.x), -0.5f * iw.y);
+                const float2 B = make_float2(0.5f * (1.f + iw.x), 0.5f * iw.y);
+                o[(int64_t)k * es] = cadd(cmul(zk, A), cmul(cconj(zm), B));
+            }
+            if (k2 != k) {
+                // bin L - k: Z[L-k] and conj Z[k] (Z[L] = Z[0])
+                const float2 iw = mul_si<1>(a.rtw[k2]);
+                const float2 A = make_float2(0.5f * (1.f - iw.x), -0.5f * iw.y);
+                const float2 B = make_float2(0.5f * (1.f + iw.x), 0.5f * iw.y);
+                o[(int64_t)k2 * es] = cadd(cmul(zm, A), cmul(cconj(zk), B));
+            }
         }
     } else if (a.io_mode == IO_C2R_FULL) {
         float *dst = static_cast<float *>(a.out);
