@@ -141,7 +141,9 @@ def bench_batching(torch, S, stream, reps=5):
     assert bh.make_plan(lengths, batches, ins, outs, S.ASCEND_FFT_FORWARD, 0) == 0
     bh.alloc_workspace()
     assert bh.set_stream(stream.cuda_stream) == 0
-    ms_b = run(lambda: bh.exec())
+    def exec_b():
+        assert bh.exec() == 0
+    ms_b = run(exec_b)
     bh.destroy()
     # (c) grouped + autotuned + one hipGraph launch
     cache = os.path.join(tempfile.gettempdir(), "sip_fft_bench_tune.jsonl")
@@ -152,7 +154,9 @@ def bench_batching(torch, S, stream, reps=5):
     tune_s = time.perf_counter() - t0
     bg.alloc_workspace()
     assert bg.set_stream(stream.cuda_stream) == 0
-    ms_c = run(lambda: bg.exec())
+    def exec_c():
+        assert bg.exec() == 0
+    ms_c = run(exec_c)
     d = bg.describe()
     bg.destroy()
     for name, ms in (("per_problem_plain", ms_a), ("grouped_direct", ms_b), ("graph_autotuned", ms_c)):
@@ -254,6 +258,16 @@ def bench_serving(torch, S, seconds, rates=(2000.0, 10000.0, 50000.0)):
     del pool
     torch.cuda.empty_cache()
     return res
+
+
+def settle(torch, leg):
+    """Surfaces a pending device error right after the leg that left it (a small launch checks it)."""
+    torch.cuda.synchronize()
+    try:
+        torch.zeros(1, device="cuda").add_(1)
+        torch.cuda.synchronize()
+    except Exception as e:  # noqa: BLE001
+        raise RuntimeError(f"device error pending after the {leg} leg: {e}") from e
 
 
 def main():
@@ -361,12 +375,15 @@ def main():
             hh.destroy()
             del xi, yo
         out["secondary"] = sec
+        settle(torch, "secondary")
 
     if not args.no_precision:
         out["precision"] = bench_precision(torch, S, stream, args.steps)
+        settle(torch, "precision")
 
     if not args.no_batching:
         out["batching"] = bench_batching(torch, S, stream)
+        settle(torch, "batching")
 
     if not args.no_serving:
         out["serving"] = bench_serving(torch, S, args.serving_seconds)

@@ -422,7 +422,7 @@ struct Builder {
             if (f <= 0) {
                 if (const char *e = std::getenv("SIP_FFT_ROW_F")) f = std::atoi(e);
             }
-            if (f >= 1 && f * T <= 1024) F = f;
+            if (f >= 1 && f * T <= 1024 && (size_t)f * per <= 160 * 1024) F = f;
         }
         if (mf) {
             // every wave owns 4 blocks of 16 butterflies: F * T must be a multiple of 64

@@ -111,3 +111,20 @@ def test_batching_workload_shape():
     sm = set(S.smooth_lengths(64, 65536))
     assert len(lengths) == 4096 and all(n in sm for n in lengths) and all(1 <= b <= 64 for b in batches)
     assert lengths == S.batching_workload(4096)[0]
+
+
+def test_autotune_leaves_no_pending_device_error(torch_cuda, tmp_path, monkeypatch):
+    """Tuning tries tile shapes that the device may refuse (too much LDS, too many threads); the
+    refusals are skipped and must not surface in the caller's next HIP call."""
+    torch = torch_cuda
+    monkeypatch.setenv("SIP_FFT_TUNE_CACHE", str(tmp_path / "tune.jsonl"))
+    lengths = [8192, 16384, 12000, 15000, 6000, 3000, 2400, 1080, 65536, 43200]
+    batches = [4, 2, 3, 2, 5, 6, 8, 9, 1, 1]
+    hs, xs, ys = _problems(torch, lengths, batches, 900)
+    bh = _run_batch(torch, lengths, batches, xs, ys, FWD, S.BATCH_GRAPH | S.BATCH_AUTOTUNE)
+    torch.zeros(1, device="cuda").add_(1)   # raises if a HIP error is pending
+    torch.cuda.synchronize()
+    for i in range(len(lengths)):
+        rel, _ = report(f"tuned problem {lengths[i]}", ys[i].cpu().numpy(), oracle.c2c(hs[i]))
+        assert rel <= TOL
+    bh.destroy()
