@@ -50,19 +50,27 @@ static float2 root_f(int64_t m, int64_t n, int sign)
     return r;
 }
 
-static TableRef pass_twiddles(const StockhamInstance *in, int sign)
+// mfma: the instance runs its radix-16 passes on the matrix cores, whose lanes take every
+// W^{k r}, r = 0..15, of a pass from a [k][16] table.
+static TableRef pass_twiddles(const StockhamInstance *in, int sign, bool mfma = false)
 {
     TableRef t;
-    t.key = "twe:" + std::to_string(in->L);
+    t.key = (mfma ? "twm:" : "twe:") + std::to_string(in->L);
     for (int p = 0; p < in->npass; p++) t.key += "," + std::to_string(in->radix[p]);
     t.key += ":" + std::to_string(sign);
-    t.gen = [in, sign]() {
+    t.gen = [in, sign, mfma]() {
         std::vector<float2> v;
         int64_t ns = 1;
         const int64_t L = in->L;
         for (int p = 0; p < in->npass; p++) {
             const int64_t R = in->radix[p];
             const int64_t step = L / (ns * R);
+            if (mfma && R == 16) {
+                for (int64_t k = 0; k < ns; k++)
+                    for (int64_t r = 0; r < 16; r++) v.push_back(root_f(k * r * step, L, sign));
+                ns *= R;
+                continue;
+            }
             // power-of-two radix: k-major [k][W^1, W^2, W^4, W^8] (radix 8 pads the fourth slot,
             // radix 4 stores two, radix 2 one and a pad; the kernel multiplies the other r
             // together); other radices: row-major [r-1][k] over every r = 1 .. R-1. Every pass
@@ -475,7 +483,7 @@ struct Builder {
         a.split_lds = rows && (p.tune.split_lds >= 0 ? p.tune.split_lds == 1
                                                      : env_flag("SIP_FFT_SPLIT_LDS", T >= 32));
         a.xcd_remap = p.tune.xcd >= 0 ? p.tune.xcd == 1 : env_flag("SIP_FFT_XCD", false);
-        st.tw = pass_twiddles(inst, sign);
+        st.tw = pass_twiddles(inst, sign, mf != nullptr);
         if (mf) st.mfw = mfma_dft_table(p.precision, sign);
         if (io_mode == IO_R2C_EVEN || io_mode == IO_C2R_EVEN) st.rtw = real_roots(L, sign);
         if (tw4_ntot) {

@@ -58,9 +58,9 @@ struct MfOp<MF_BF16> {
     typedef mf_short4 vec;
     static __device__ __forceinline__ unsigned short to_bits(float x)
     {
-        // round to nearest even (finite inputs)
-        const unsigned u = __float_as_uint(x);
-        return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+        // round to nearest even (the gfx950 conversion instruction)
+        const __bf16 h = (__bf16)x;
+        return __builtin_bit_cast(unsigned short, h);
     }
     static __device__ __forceinline__ float from_bits(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
     static __device__ __forceinline__ mf_float4 mma(vec a, vec b, mf_float4 c)
@@ -116,6 +116,15 @@ __device__ __forceinline__ void mf_split(const float (&x)[4], float scale, typen
     for (int tm = 0; tm < NT; tm++) out[tm] = MfOp<DT>::pack(b[tm]);
 }
 
+// Twiddle rows of a pass in an MFMA chain: radix-16 passes keep every W^{k r}, r = 0..15, k-major
+// ([k][16]), so that each lane loads the four it multiplies (r = 4 gq .. 4 gq + 3) with two 16-byte
+// loads; the VALU passes keep the plain layout.
+template <int R>
+constexpr int mf_tw_rows()
+{
+    return R == 16 ? 16 : tw_rows<R>();
+}
+
 // One radix-16 pass on MFMA for a tile of F transforms of length L in LDS (T = L/16 threads per
 // transform, F * T a multiple of 64). NS = product of the earlier radices. GIN: the pass reads its
 // inputs from HBM (src + in_base[f]); GOUT: it writes its outputs to HBM (dst + out_base[f]).
@@ -131,7 +140,9 @@ __device__ __forceinline__ void pass_mfma16(float *re, float *im, const float2 *
     const int j = lane & 15, gq = lane >> 4;
     float xr[4][4], xi[4][4];
     int fls[4], bs[4];
-    // gather: block u of this wave covers global butterflies (4 wave + u) * 16 + 0..15
+    // gather: block u of this wave covers global butterflies (4 wave + u) * 16 + 0..15. Every lane
+    // gathers (fl < F always, so LDS reads stay inside the tile; HBM reads of transforms beyond nf
+    // re-read the last valid one); only the stores are guarded.
 #pragma unroll
     for (int u = 0; u < 4; u++) {
         const int g = (wave * 4 + u) * 16 + j;
@@ -139,46 +150,36 @@ __device__ __forceinline__ void pass_mfma16(float *re, float *im, const float2 *
         const int b = g - fl * NBF;
         fls[u] = fl;
         bs[u] = b;
-        const bool act = fl < nf;
-        const int lbase = (act ? fl : 0) * LP;
-        // twiddles W^{k r} of this lane's rows r = 4 gq + t from the stored powers W^1, W^2, W^4, W^8:
-        // W^r = W^{4 gq} W^t
+        // this lane's twiddles W^{k r}, r = 4 gq + t, straight from the [k][16] pass table
         float2 lane_tw[4];
         if constexpr (NS > 1) {
-            if (act) {
-                float2 wl[16];
-                tw_load<16, NS>(tw, b % NS, wl);
-                const float2 w1 = wl[1], w2 = wl[2], w4 = wl[4], w8 = wl[8];
-                const float2 w3 = cmul(w1, w2);
-                const float2 one = make_float2(1.f, 0.f);
-                const float2 hi = gq == 0 ? one : gq == 1 ? w4 : gq == 2 ? w8 : cmul(w4, w8);
-                lane_tw[0] = hi;
-                lane_tw[1] = gq == 0 ? w1 : cmul(hi, w1);
-                lane_tw[2] = gq == 0 ? w2 : cmul(hi, w2);
-                lane_tw[3] = gq == 0 ? w3 : cmul(hi, w3);
-            }
+            const float4 *tp = reinterpret_cast<const float4 *>(tw + (b % NS) * 16 + 4 * gq);
+            const float4 p0 = tp[0], p1 = tp[1];
+            lane_tw[0] = make_float2(p0.x, p0.y);
+            lane_tw[1] = make_float2(p0.z, p0.w);
+            lane_tw[2] = make_float2(p1.x, p1.y);
+            lane_tw[3] = make_float2(p1.z, p1.w);
         }
+        const float2 *gsrc = nullptr;
+        if constexpr (GIN) gsrc = src + in_base[fl < nf ? fl : nf - 1] + b;
+        const int lbase = fl * LP;
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             const int r = 4 * gq + t;
-            float vr = 0.f, vi = 0.f;
-            if (act) {
-                if constexpr (GIN) {
-                    const float2 v = gload(src + in_base[fl] + b + r * NBF);
-                    vr = v.x;
-                    vi = v.y;
-                } else {
-                    const int jl = lds_idx_read<NBF>(b, r);
-                    vr = re[lbase + jl];
-                    vi = im[lbase + jl];
-                }
-                if constexpr (NS > 1) {
-                    if (r > 0) {
-                        const float2 p = cmul(make_float2(vr, vi), lane_tw[t]);
-                        vr = p.x;
-                        vi = p.y;
-                    }
-                }
+            float vr, vi;
+            if constexpr (GIN) {
+                const float2 v = gload(gsrc + r * NBF);
+                vr = v.x;
+                vi = v.y;
+            } else {
+                const int jl = lds_idx_read<NBF>(b, r);
+                vr = re[lbase + jl];
+                vi = im[lbase + jl];
+            }
+            if constexpr (NS > 1) {
+                const float2 p = cmul(make_float2(vr, vi), lane_tw[t]);  // W^0 = 1 exactly for r = 0
+                vr = p.x;
+                vi = p.y;
             }
             xr[u][t] = vr;
             xi[u][t] = vi;
@@ -190,13 +191,24 @@ __device__ __forceinline__ void pass_mfma16(float *re, float *im, const float2 *
         float scale = 1.f, unscale = 1.f;
         if constexpr (DT == MF_F16) {
             // per-column power-of-two scale: the column lives in lanes j, j+16, j+32, j+48
-            float m = 0.f;
+            // max |x| as bits (unsigned order = float order for non-negative floats)
+            unsigned mb = 0;
 #pragma unroll
-            for (int t = 0; t < 4; t++) m = fmaxf(m, fmaxf(fabsf(xr[u][t]), fabsf(xi[u][t])));
-            m = fmaxf(m, __shfl_xor(m, 16));
-            m = fmaxf(m, __shfl_xor(m, 32));
-            if (m > 0.f) {
-                int e = (int)((__float_as_uint(m) >> 23) & 255u) - 127;  // m in [2^e, 2^(e+1))
+            for (int t = 0; t < 4; t++)
+                mb = max(mb, max(__float_as_uint(xr[u][t]) & 0x7FFFFFFFu, __float_as_uint(xi[u][t]) & 0x7FFFFFFFu));
+            // v_permlane16_swap with both operands = mb leaves rows 2i and 2i+1 of the two results
+            // holding each other's values, v_permlane32_swap the two half-waves': two maxima give
+            // every lane the maximum over lanes j, j+16, j+32, j+48
+            {
+                const auto r = __builtin_amdgcn_permlane16_swap(mb, mb, false, false);
+                mb = max((unsigned)r[0], (unsigned)r[1]);
+            }
+            {
+                const auto r = __builtin_amdgcn_permlane32_swap(mb, mb, false, false);
+                mb = max((unsigned)r[0], (unsigned)r[1]);
+            }
+            if (mb != 0u) {
+                int e = (int)(mb >> 23) - 127;  // max |x| in [2^e, 2^(e+1))
                 e = e < -100 ? -100 : (e > 100 ? 100 : e);
                 scale = __uint_as_float((unsigned)(127 + 13 - e) << 23);
                 unscale = __uint_as_float((unsigned)(127 - 13 + e) << 23);
@@ -268,7 +280,7 @@ __device__ __forceinline__ void passes_mixed(float *re, float *im, const float2 
     else
         pass_static<L, T, S, NS, R, DIN && first, DOUT && last>(re, im, tw, lbase, t, active, RowIo{io.gin, io.gout});
     if constexpr (!last)
-        passes_mixed<L, T, S, DT, NT, DIN, DOUT, NS * R, Rest...>(re, im, tw + NS * tw_rows<R>(), W, LP, nf, lbase, t,
+        passes_mixed<L, T, S, DT, NT, DIN, DOUT, NS * R, Rest...>(re, im, tw + NS * mf_tw_rows<R>(), W, LP, nf, lbase, t,
                                                                  active, io);
 }
 
