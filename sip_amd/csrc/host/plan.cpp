@@ -71,24 +71,17 @@ static TableRef pass_twiddles(const StockhamInstance *in, int sign, bool mfma = 
                 ns *= R;
                 continue;
             }
-            // power-of-two radix: k-major [k][W^1, W^2, W^4, W^8] (radix 8 pads the fourth slot,
-            // radix 4 stores two, radix 2 one and a pad; the kernel multiplies the other r
-            // together); other radices: row-major [r-1][k] over every r = 1 .. R-1. Every pass
-            // holds an even number of entries per k, so each pass table stays 16-byte aligned.
-            if ((R & (R - 1)) == 0) {
-                const int slots = R <= 4 ? 2 : 4;
-                if (R > 16) throw SipError(AsdSip::ACL_ERROR_INTERNAL_ERROR, "radix above 16 in a leaf");
-                for (int64_t k = 0; k < ns; k++)
-                    for (int sl = 0; sl < slots; sl++) {
-                        const int64_t r = int64_t(1) << sl;
-                        v.push_back(r < R ? root_f(k * r * step, L, sign) : float2{0.f, 0.f});
-                    }
-            } else {
-                for (int64_t r = 1; r < R; r++)
-                    for (int64_t k = 0; k < ns; k++) v.push_back(root_f(k * r * step, L, sign));
-                if ((R - 1) % 2)  // pad to an even row count (16-byte aligned pass tables)
-                    for (int64_t k = 0; k < ns; k++) v.push_back(float2{0.f, 0.f});
-            }
+            // k-major [k][W^1, W^2, W^4, ..] over the powers of two below R, padded to an even
+            // count (the kernel multiplies the other r together); each pass table stays 16-byte
+            // aligned.
+            int nl = 0;
+            while ((int64_t(1) << nl) < R) nl++;
+            const int slots = (nl + 1) & ~1;
+            for (int64_t k = 0; k < ns; k++)
+                for (int sl = 0; sl < slots; sl++) {
+                    const int64_t r = int64_t(1) << sl;
+                    v.push_back(sl < nl ? root_f(k * r * step, L, sign) : float2{0.f, 0.f});
+                }
             ns *= R;
         }
         if (v.empty()) v.push_back(root_f(0, 1, 1));

@@ -88,59 +88,48 @@ __device__ __forceinline__ int lds_idx_write(int hi, int k, int r)
     }
 }
 
-// Twiddle rows of a radix-R pass. A power-of-two radix stores W^{k r} for r = 1, 2, 4, .. R/2 only
-// (the other r are products of those, at most three multiplies deep), k-major and padded to an
-// even count, so a radix-16 or radix-8 pass loads its twiddles with two 16-byte loads: layout
-// [k][4] (radix 8 and 16), [k][2] (radix 4; radix 2 uses the first slot). Other radices store
-// every r = 1 .. R-1 row-major, [r-1][k], padded to an even row count. Every pass has an even
-// number of rows, so every pass table starts 16-byte aligned.
+// Twiddle rows of a radix-R pass: only W^{k r} for the powers of two r = 1, 2, 4, .. < R are stored
+// (the other r are products of those: w[r] = w[highest power of two in r] * w[rest], at most four
+// multiplies deep for r < 32), k-major and padded to an even count, so that a pass loads its
+// twiddles with one to three 16-byte loads: [k][2] (radix 2, 3, 4), [k][4] (5 .. 16), [k][6] (20).
+// Every pass has an even number of rows, so every pass table starts 16-byte aligned.
+template <int R>
+constexpr int tw_nlog()
+{
+    int n = 0;
+    while ((1 << n) < R) n++;
+    return n;
+}
+
 template <int R>
 constexpr int tw_rows()
 {
-    if constexpr (R == 2 || R == 4) {
-        return 2;
-    } else if constexpr (R == 8 || R == 16) {
-        return 4;
-    } else {
-        static_assert((R & (R - 1)) != 0, "power-of-two radices above 16 have no twiddle layout");
-        return (R - 1 + 1) & ~1;
-    }
+    return (tw_nlog<R>() + 1) & ~1;
 }
 
 template <int R, int NS>
 __device__ __forceinline__ void tw_load(const float2 *__restrict__ tw, int k, float2 (&w)[R])
 {
-    if constexpr (R == 2) {
-        w[1] = tw[2 * k];
-    } else if constexpr (R == 4) {
-        const float4 a = reinterpret_cast<const float4 *>(tw)[k];
-        w[1] = make_float2(a.x, a.y);
-        w[2] = make_float2(a.z, a.w);
-    } else if constexpr (R == 8 || R == 16) {
-        const float4 a = reinterpret_cast<const float4 *>(tw)[2 * k];
-        const float4 b = reinterpret_cast<const float4 *>(tw)[2 * k + 1];
-        w[1] = make_float2(a.x, a.y);
-        w[2] = make_float2(a.z, a.w);
-        w[4] = make_float2(b.x, b.y);
-        if constexpr (R == 16) w[8] = make_float2(b.z, b.w);
-    } else {
+    constexpr int NL = tw_nlog<R>(), NR = tw_rows<R>();
+    const float4 *p = reinterpret_cast<const float4 *>(tw + k * NR);
 #pragma unroll
-        for (int r = 1; r < R; r++) w[r] = tw[(r - 1) * NS + k];
+    for (int i = 0; i < NR / 2; i++) {
+        const float4 a = p[i];
+        if (2 * i < NL) w[1 << (2 * i)] = make_float2(a.x, a.y);
+        if (2 * i + 1 < NL) w[1 << (2 * i + 1)] = make_float2(a.z, a.w);
     }
 }
 
-// Completes w[1 .. R-1] from the loaded powers (power-of-two radices).
+// Completes w[1 .. R-1] from the loaded powers of two.
 template <int R>
 __device__ __forceinline__ void tw_expand(float2 (&w)[R])
 {
-    if constexpr ((R & (R - 1)) == 0) {
 #pragma unroll
-        for (int r = 3; r < R; r++) {
-            if ((r & (r - 1)) == 0) continue;
-            int hb = 1;
-            while (hb * 2 <= r) hb *= 2;
-            w[r] = cmul(w[hb], w[r - hb]);
-        }
+    for (int r = 3; r < R; r++) {
+        if ((r & (r - 1)) == 0) continue;
+        int hb = 1;
+        while (hb * 2 <= r) hb *= 2;
+        w[r] = cmul(w[hb], w[r - hb]);
     }
 }
 
