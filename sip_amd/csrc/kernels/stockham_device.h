@@ -476,6 +476,65 @@ __device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds,
                 im[fl * LP + lds_pad(j)] = 0.f;
             }
         }
+    } else if (a.io_mode == IO_C2R_EVEN) {
+        // Half spectrum X[0..L] straight from HBM into the pre-processed L-point input
+        // P[k] = (X[k] + conj X[L-k]) + i w^k (X[k] - conj X[L-k]) (imag of X[0] and X[L] dropped).
+        // Unit kk > 0 of a transform reads X[kk] and X[L-kk] and writes P[kk] and P[L-kk]
+        // (w^{L-k} = -conj w^k); unit 0 writes P[0]. For even L the self-paired P[L/2] is one more
+        // step per transform. The block is F * T threads and a transform has (L+1)/2 units, so a
+        // thread has at most NU; all loads come first.
+        constexpr int H = L / 2;
+        constexpr int NUN = (L + 1) / 2;
+        constexpr bool EVEN = (L % 2) == 0;
+        constexpr int NU = (NUN + T - 1) / T;
+        const float2 *src = static_cast<const float2 *>(a.in);
+        const int64_t es = a.tin.es;
+        float2 xk[NU], xm[NU], wk[NU];
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+            const int i = tid + u * nthreads;
+            const int fl = i / NUN;
+            const int kk = i - fl * NUN;
+            if (i < F * NUN && fl < nf) {
+                const float2 *x = src + in_base[fl];
+                xk[u] = x[(int64_t)kk * es];
+                xm[u] = x[(int64_t)(L - kk) * es];
+                wk[u] = a.rtw[kk];
+            }
+        }
+        if (EVEN && tid < nf) {
+            const float2 xh = src[in_base[tid] + (int64_t)H * es];
+            const float2 c = cconj(xh);
+            const float2 p = cadd(cadd(xh, c), mul_si<1>(cmul(a.rtw[H], csub(xh, c))));
+            re[tid * LP + lds_pad(H)] = p.x;
+            im[tid * LP + lds_pad(H)] = p.y;
+        }
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+            const int i = tid + u * nthreads;
+            const int fl = i / NUN;
+            const int kk = i - fl * NUN;
+            if (i >= F * NUN || fl >= nf) continue;
+            float *pr = re + fl * LP, *pi = im + fl * LP;
+            float2 a0 = xk[u], a1 = xm[u];
+            if (kk == 0) {
+                a0.y = 0.f;
+                a1.y = 0.f;
+            }
+            {
+                const float2 c = cconj(a1);
+                const float2 p = cadd(cadd(a0, c), mul_si<1>(cmul(wk[u], csub(a0, c))));
+                pr[lds_pad(kk)] = p.x;
+                pi[lds_pad(kk)] = p.y;
+            }
+            if (kk != 0) {
+                const float2 w2 = make_float2(-wk[u].x, wk[u].y);
+                const float2 c = cconj(a0);
+                const float2 p = cadd(cadd(a1, c), mul_si<1>(cmul(w2, csub(a1, c))));
+                pr[lds_pad(L - kk)] = p.x;
+                pi[lds_pad(L - kk)] = p.y;
+            }
+        }
     } else if (a.tin.vec2) {
         // the block is F * T threads: every thread moves at most NV float4 (all loads in flight
         // before the first LDS store); in_len is at most L + 1 (C2R half spectrum)
@@ -564,55 +623,8 @@ __device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds,
     }
     __syncthreads();
 
-    // ---------------------------------------------------------------- pre-process (C2R)
-    if (a.io_mode == IO_C2R_EVEN) {
-        // P[k] = (X[k] + conj X[L-k]) + i w^k (X[k] - conj X[L-k]); imag of X[0] and X[L] dropped.
-        // The block is F * T threads: a thread handles at most NP pairs; the root loads of all of
-        // them are issued before the first LDS access.
-        constexpr int NPAIR = L / 2 + 1;
-        constexpr int NP = (NPAIR + T - 1) / T;
-        float2 w1[NP], w2[NP];
-#pragma unroll
-        for (int u = 0; u < NP; u++) {
-            const int i = tid + u * nthreads;
-            const int fl = i / NPAIR;
-            const int k = i - fl * NPAIR;
-            if (i < F * NPAIR && fl < nf) {
-                w1[u] = a.rtw[k];
-                w2[u] = a.rtw[L - k];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < NP; u++) {
-            const int i = tid + u * nthreads;
-            const int fl = i / NPAIR;
-            const int k = i - fl * NPAIR;
-            if (i >= F * NPAIR || fl >= nf) continue;
-            float *pr = re + fl * LP, *pi = im + fl * LP;
-            const int k2 = L - k;
-            float2 xk = make_float2(pr[lds_pad(k)], pi[lds_pad(k)]);
-            float2 xm = make_float2(pr[lds_pad(k2)], pi[lds_pad(k2)]);
-            if (k == 0) {
-                xk.y = 0.f;
-                xm.y = 0.f;
-            }
-            {
-                const float2 c = cconj(xm);
-                const float2 s = cadd(xk, c), d = csub(xk, c);
-                const float2 p = cadd(s, mul_si<1>(cmul(w1[u], d)));
-                pr[lds_pad(k)] = p.x;
-                pi[lds_pad(k)] = p.y;
-            }
-            if (k != 0 && k2 != k) {
-                const float2 c = cconj(xk);
-                const float2 s = cadd(xm, c), d = csub(xm, c);
-                const float2 p = cadd(s, mul_si<1>(cmul(w2[u], d)));
-                pr[lds_pad(k2)] = p.x;
-                pi[lds_pad(k2)] = p.y;
-            }
-        }
-        __syncthreads();
-    } else if (a.io_mode == IO_C2R_FULL) {
+    // ---------------------------------------------------------------- pre-process (C2R, odd)
+    if (a.io_mode == IO_C2R_FULL) {
         // Hermitian extension of bins 0..L/2 to L points; imag of bin 0 (and L/2) dropped.
         constexpr int H = L / 2;
         for (int i = tid; i < F * L; i += nthreads) {
@@ -677,34 +689,51 @@ __device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds,
             }
         }
     } else if (a.io_mode == IO_R2C_EVEN) {
-        // X[k] = Z[k] (1 - i w^k)/2 + conj Z[L-k] (1 + i w^k)/2, k = 0..L. The bins k and L - k
-        // use the same two LDS values: one thread computes both (pairs k = 0..L/2).
+        // X[k] = Z[k] (1 - i w^k)/2 + conj Z[L-k] (1 + i w^k)/2, k = 0..L (Z[L] = Z[0]). Unit kk > 0
+        // of a transform writes bins kk and L - kk from the same two LDS values (w^{L-k} =
+        // -conj w^k); unit 0 writes bins 0 and L. For even L the self-paired bin L/2 is one more
+        // step per transform. A transform has (L+1)/2 units and the block F * T threads; all LDS
+        // and root loads come before the first store.
+        constexpr int H = L / 2;
+        constexpr int NUN = (L + 1) / 2;
+        constexpr bool EVEN = (L % 2) == 0;
+        constexpr int NU = (NUN + T - 1) / T;
         float2 *dst = static_cast<float2 *>(a.out);
-        constexpr int NPAIR = L / 2 + 1;
-        for (int i = tid; i < F * NPAIR; i += nthreads) {
-            const int fl = i / NPAIR;
-            const int k = i - fl * NPAIR;
-            if (fl >= nf) continue;
-            const float *pr = re + fl * LP, *pi = im + fl * LP;
-            const int k2 = L - k;
-            const int ka = k, kb = (k == 0) ? 0 : k2;
-            const float2 zk = make_float2(pr[lds_pad(ka)], pi[lds_pad(ka)]);
-            const float2 zm = make_float2(pr[lds_pad(kb)], pi[lds_pad(kb)]);
+        const int64_t es = a.tout.es;
+        float2 zk[NU], zm[NU], wk[NU];
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+            const int i = tid + u * nthreads;
+            const int fl = i / NUN;
+            const int kk = i - fl * NUN;
+            if (i < F * NUN && fl < nf) {
+                const float *pr = re + fl * LP, *pi = im + fl * LP;
+                const int km = kk == 0 ? 0 : L - kk;
+                zk[u] = make_float2(pr[lds_pad(kk)], pi[lds_pad(kk)]);
+                zm[u] = make_float2(pr[lds_pad(km)], pi[lds_pad(km)]);
+                wk[u] = a.rtw[kk];
+            }
+        }
+        auto bin = [](float2 z, float2 zc, float2 w) {
+            const float2 iw = mul_si<1>(w);
+            const float2 A = make_float2(0.5f * (1.f - iw.x), -0.5f * iw.y);
+            const float2 B = make_float2(0.5f * (1.f + iw.x), 0.5f * iw.y);
+            return cadd(cmul(z, A), cmul(cconj(zc), B));
+        };
+        if (EVEN && tid < nf) {
+            const float2 zh = make_float2(re[tid * LP + lds_pad(H)], im[tid * LP + lds_pad(H)]);
+            dst[out_base[tid] + (int64_t)H * es] = bin(zh, zh, a.rtw[H]);
+        }
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+            const int i = tid + u * nthreads;
+            const int fl = i / NUN;
+            const int kk = i - fl * NUN;
+            if (i >= F * NUN || fl >= nf) continue;
             float2 *o = dst + out_base[fl];
-            const int64_t es = a.tout.es;
-            {
-                const float2 iw = mul_si<1>(a.rtw[k]);
-                const float2 A = make_float2(0.5f * (1.f - iw.x), -0.5f * iw.y);
-                const float2 B = make_float2(0.5f * (1.f + iw.x), 0.5f * iw.y);
-                o[(int64_t)k * es] = cadd(cmul(zk, A), cmul(cconj(zm), B));
-            }
-            if (k2 != k) {
-                // bin L - k: Z[L-k] and conj Z[k] (Z[L] = Z[0])
-                const float2 iw = mul_si<1>(a.rtw[k2]);
-                const float2 A = make_float2(0.5f * (1.f - iw.x), -0.5f * iw.y);
-                const float2 B = make_float2(0.5f * (1.f + iw.x), 0.5f * iw.y);
-                o[(int64_t)k2 * es] = cadd(cmul(zm, A), cmul(cconj(zk), B));
-            }
+            const float2 w2 = make_float2(-wk[u].x, wk[u].y);  // w^{L - kk}
+            o[(int64_t)kk * es] = bin(zk[u], zm[u], wk[u]);
+            o[(int64_t)(L - kk) * es] = bin(zm[u], zk[u], w2);
         }
     } else if (a.io_mode == IO_C2R_FULL) {
         float *dst = static_cast<float *>(a.out);
