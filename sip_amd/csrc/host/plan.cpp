@@ -475,6 +475,13 @@ struct Builder {
         // available as the tuning switch SIP_FFT_XCD=1.
         a.split_lds = rows && (p.tune.split_lds >= 0 ? p.tune.split_lds == 1
                                                      : env_flag("SIP_FFT_SPLIT_LDS", T >= 32));
+        // Column tiles with direct HBM passes of power-of-two length: single plane too (a 1024 x 16
+        // tile needs 70 KiB instead of 139 KiB, so two workgroups fit on a CU). Measured on
+        // MI355X: 2D 1024^2 x 128 892 -> 863 us, 1024^2 x 8 52 -> 48 us, 512^2 x 256 445 -> 422 us,
+        // 2^20 x 64 606 -> 566 us; 1000^2 x 64 lost (482 -> 506 us), so mixed lengths keep two
+        // planes.
+        if (a.col_direct && inst->cols_fwd != nullptr && (L & (L - 1)) == 0 && env_flag("SIP_FFT_SPLIT_COLS", true))
+            a.split_lds = 1;
         a.xcd_remap = p.tune.xcd >= 0 ? p.tune.xcd == 1 : env_flag("SIP_FFT_XCD", false);
         st.tw = pass_twiddles(inst, sign, mf != nullptr);
         if (mf) st.mfw = mfma_dft_table(p.precision, sign);
@@ -940,6 +947,7 @@ Plan::~Plan()
 static const void *stage_kernel(const Stage &s)
 {
     if (s.inst2d) return s.sign < 0 ? s.inst2d->fwd : s.inst2d->inv;
+    if (s.sa.split_lds && s.sa.col_direct) return s.sign < 0 ? s.inst->cols_fwd : s.inst->cols_inv;
     if (s.sa.split_lds) return s.sign < 0 ? s.inst->rows_fwd : s.inst->rows_inv;
     return s.sign < 0 ? s.inst->fwd : s.inst->inv;
 }

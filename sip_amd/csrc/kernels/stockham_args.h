@@ -102,6 +102,8 @@ struct StockhamInstance {
     const void *inv;
     const void *rows_fwd;    // single-plane contiguous-row kernels (T >= 16 only, else null)
     const void *rows_inv;
+    const void *cols_fwd;    // single-plane column-tile kernels (L <= 1024 only, else null)
+    const void *cols_inv;
 };
 
 // Registry entry of a fused 2D tile instance (stockham2d.h). The radix sequences, and so the

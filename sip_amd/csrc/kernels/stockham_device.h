@@ -241,12 +241,13 @@ __device__ __forceinline__ float2 tw4_root(const StockhamArgs &a, int64_t m)
     return cmul(a.tw4_lo[m & a.tw4_mask], a.tw4_hi[m >> a.tw4_shift]);
 }
 
-// Contiguous C2C rows with one LDS plane per transform (half the LDS of the two-plane layout, so
-// twice as many workgroups fit on a CU). The first pass reads HBM, the last writes HBM, and every
-// exchange in between moves the real parts through the plane, then the imaginary parts.
-template <int L, int T, int S, int NS, int R, int... Rest>
+// C2C transforms with one LDS plane per transform (half the LDS of the two-plane layout, so
+// twice as many workgroups fit on a CU): contiguous rows, or column tiles whose lanes walk adjacent
+// columns. The first pass reads HBM, the last writes HBM through io.store, and every exchange in
+// between moves the real parts through the plane, then the imaginary parts.
+template <int L, int T, int S, int NS, int R, int... Rest, class Io>
 __device__ __forceinline__ void split_chain(float *pl, const float2 *__restrict__ tw, int lbase, int t, bool active,
-                                            float2 (&v)[(L / R + T - 1) / T][R], float2 *__restrict__ gout)
+                                            float2 (&v)[(L / R + T - 1) / T][R], const Io &io)
 {
     constexpr int NBF = L / R;
     constexpr int NB = (NBF + T - 1) / T;
@@ -274,7 +275,7 @@ __device__ __forceinline__ void split_chain(float *pl, const float2 *__restrict_
                 const int k = b % NS;
                 const int d = (b / NS) * NS * R + k;
 #pragma unroll
-                for (int r = 0; r < R; r++) gstore(gout + d + r * NS, v[q][r]);
+                for (int r = 0; r < R; r++) io.store(d + r * NS, v[q][r]);
             }
         }
     } else {
@@ -313,7 +314,7 @@ __device__ __forceinline__ void split_chain(float *pl, const float2 *__restrict_
                 }
             }
         }
-        split_chain<L, T, S, NS * R, Rest...>(pl, tw + NS * tw_rows<R>(), lbase, t, active, n, gout);
+        split_chain<L, T, S, NS * R, Rest...>(pl, tw + NS * tw_rows<R>(), lbase, t, active, n, io);
     }
 }
 
@@ -344,7 +345,8 @@ __device__ __forceinline__ void split_rows(const StockhamArgs &a, float *pl, int
             for (int r = 0; r < R0; r++) v[q][r] = gload(g + b + r * NBF0);
         }
     }
-    split_chain<L, T, S, 1, R0, Rest...>(pl, a.tw, flp * a.LP, t, active, v, static_cast<float2 *>(a.out) + ob);
+    split_chain<L, T, S, 1, R0, Rest...>(pl, a.tw, flp * a.LP, t, active, v,
+                                         RowIo{nullptr, static_cast<float2 *>(a.out) + ob});
 }
 
 // Tile of this workgroup; with xcd_remap the round-robin dispatch of consecutive workgroups to the
@@ -383,6 +385,50 @@ struct ColIo {
         gstore(out + (int64_t)j * es_out, v);
     }
 };
+
+// Column tile with one LDS plane per transform: lane tid walks transform tid mod F (adjacent
+// columns, so a wave's HBM accesses cover whole lines) and butterflies tid / F; the first pass
+// reads and the last pass writes HBM through ColIo (with the four-step twiddle on store).
+template <int L, int T, int S, int R0, int... Rest>
+__device__ __forceinline__ void split_cols(const StockhamArgs &a, float *pl, int64_t tile)
+{
+    constexpr int NBF0 = L / R0;
+    constexpr int NB0 = (NBF0 + T - 1) / T;
+    constexpr bool EXACT0 = (NBF0 % T) == 0;
+    const int F = a.F;
+    const int tid = threadIdx.x;
+    const int t = (int)fdiv((uint32_t)tid, a.div_F);
+    const int flp = tid - t * F;
+    const int64_t f = tile * F + flp;
+    const bool active = f < a.nfft;
+    int64_t ib = 0, ob = 0, col = 0;
+    if (active) {
+        uint32_t f0;
+        ib = tile_base((uint32_t)f, a.tin, &f0);
+        col = fdiv(f0, a.div_col);
+        ob = tile_base((uint32_t)f, a.tout);
+    }
+    const ColIo io{static_cast<const float2 *>(a.in) + ib, static_cast<float2 *>(a.out) + ob, a.tin.es, a.tout.es,
+                   col, &a};
+    float2 v[NB0][R0];
+#pragma unroll
+    for (int q = 0; q < NB0; q++) {
+        const int b = t + q * T;
+        if (active && (EXACT0 || b < NBF0)) {
+#pragma unroll
+            for (int r = 0; r < R0; r++) v[q][r] = io.load(b + r * NBF0);
+        }
+    }
+    split_chain<L, T, S, 1, R0, Rest...>(pl, a.tw, flp * a.LP, t, active, v, io);
+}
+
+// Column tiles, single LDS plane (separate kernel so that its register budget is its own).
+template <int L, int T, int S, int... Rs>
+__global__ __launch_bounds__(1024) void stockham_cols_kernel(const StockhamArgs a)
+{
+    extern __shared__ float lds[];
+    split_cols<L, T, S, Rs...>(a, lds, tile_index(a));
+}
 
 // Pass runners: the radix chain the tile goes through between load and store.
 template <int L, int T, int S, int... Rs>
