@@ -482,6 +482,11 @@ struct Builder {
         // planes.
         if (a.col_direct && inst->cols_fwd != nullptr && (L & (L - 1)) == 0 && env_flag("SIP_FFT_SPLIT_COLS", true))
             a.split_lds = 1;
+        // Fused real transforms on contiguous rows: single plane as well (pair pre-process read
+        // straight from HBM for C2R, post-process through the plane for R2C).
+        const bool real_rows = (io_mode == IO_R2C_EVEN && a.direct_in && a.tout.es == 1 && !a.tout.unit_f) ||
+                               (io_mode == IO_C2R_EVEN && a.direct_out && a.tin.es == 1 && !a.tin.unit_f);
+        if (real_rows && inst->real_rows[0] != nullptr && env_flag("SIP_FFT_SPLIT_REAL", true)) a.split_lds = 1;
         a.xcd_remap = p.tune.xcd >= 0 ? p.tune.xcd == 1 : env_flag("SIP_FFT_XCD", false);
         st.tw = pass_twiddles(inst, sign, mf != nullptr);
         if (mf) st.mfw = mfma_dft_table(p.precision, sign);
@@ -948,6 +953,8 @@ static const void *stage_kernel(const Stage &s)
 {
     if (s.inst2d) return s.sign < 0 ? s.inst2d->fwd : s.inst2d->inv;
     if (s.sa.split_lds && s.sa.col_direct) return s.sign < 0 ? s.inst->cols_fwd : s.inst->cols_inv;
+    if (s.sa.split_lds && s.sa.io_mode == IO_R2C_EVEN) return s.inst->real_rows[s.sign < 0 ? 0 : 1];
+    if (s.sa.split_lds && s.sa.io_mode == IO_C2R_EVEN) return s.inst->real_rows[s.sign < 0 ? 2 : 3];
     if (s.sa.split_lds) return s.sign < 0 ? s.inst->rows_fwd : s.inst->rows_inv;
     return s.sign < 0 ? s.inst->fwd : s.inst->inv;
 }

@@ -140,6 +140,8 @@ def instances():
 # longest leaf used for column tiles (the host's kColumnLeafMax): these lengths also get the
 # single-plane column-tile kernel
 COLUMN_LEAF_MAX = 1024
+# half lengths of the real transforms that also get single-plane R2C / C2R row kernels
+REAL_ROWS_LENGTHS = (512, 1024, 2048, 4096, 8192)
 
 
 def main():
@@ -185,7 +187,14 @@ def main():
                 cols = f"(const void *)&{cf}, (const void *)&{ci}"
             else:
                 cols = "nullptr, nullptr"
-            tab.append(f"    {{{n}, {t}, {len(seq)}, {{{radix}}}, (const void *)&{kf}, (const void *)&{ki}, {rows}, {cols}}},")
+            if n in REAL_ROWS_LENGTHS:
+                names = [f"stockham_real_rows_kernel<{n}, {t}, {sg}, {m}, {rs}>" for m in (1, 2) for sg in (-1, 1)]
+                for nm in names:
+                    lines.append(f"template __global__ void {nm}(const StockhamArgs);")
+                real = "{" + ", ".join(f"(const void *)&{nm}" for nm in names) + "}"
+            else:
+                real = "{nullptr, nullptr, nullptr, nullptr}"
+            tab.append(f"    {{{n}, {t}, {len(seq)}, {{{radix}}}, (const void *)&{kf}, (const void *)&{ki}, {rows}, {cols}, {real}}},")
         lines.append(f"extern const StockhamInstance g_stockham_part_{k}[] = {{")
         lines += tab
         lines.append("};")

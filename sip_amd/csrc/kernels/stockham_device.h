@@ -135,6 +135,7 @@ __device__ __forceinline__ void tw_expand(float2 (&w)[R])
 
 // HBM endpoints of the direct passes: element j of this thread's transform.
 struct RowIo {  // contiguous transform
+    static constexpr bool kPairTail = false;
     const float2 *__restrict__ in;
     float2 *__restrict__ out;
     __device__ __forceinline__ float2 load(int j) const { return gload(in + j); }
@@ -241,6 +242,99 @@ __device__ __forceinline__ float2 tw4_root(const StockhamArgs &a, int64_t m)
     return cmul(a.tw4_lo[m & a.tw4_mask], a.tw4_hi[m >> a.tw4_shift]);
 }
 
+// Real-transform pair arithmetic (half-length complex transform Z of the even / odd samples,
+// w^k = e^{s 2 pi i k / 2L}):
+//   R2C bin  X[k] = Z[k] (1 - i w^k)/2 + conj Z[L-k] (1 + i w^k)/2,
+//   C2R input P[k] = (X[k] + conj X[L-k]) + i w^k (X[k] - conj X[L-k]).
+// Bin / input L - k uses w^{L-k} = -conj w^k with the roles of k and L - k exchanged.
+__device__ __forceinline__ float2 r2c_bin(float2 z, float2 zc, float2 w)
+{
+    const float2 iw = mul_si<1>(w);
+    const float2 A = make_float2(0.5f * (1.f - iw.x), -0.5f * iw.y);
+    const float2 B = make_float2(0.5f * (1.f + iw.x), 0.5f * iw.y);
+    return cadd(cmul(z, A), cmul(cconj(zc), B));
+}
+__device__ __forceinline__ float2 c2r_input(float2 x, float2 xc, float2 w)
+{
+    const float2 c = cconj(xc);
+    return cadd(cadd(x, c), mul_si<1>(cmul(w, csub(x, c))));
+}
+
+// R2C end of a single-plane chain: the last pass's outputs Z (natural order) pass through the
+// plane to the post-process units (unit kk > 0: bins kk and L - kk from Z[kk], Z[L-kk]; unit 0:
+// bins 0 and L from Z[0]; for even L thread 0 also writes the self-paired bin L/2).
+struct R2cTail {
+    static constexpr bool kPairTail = true;
+    float2 *__restrict__ out;  // this transform's L + 1 bins
+    const float2 *__restrict__ rtw;
+};
+
+template <int L, int T, int NS, int R>
+__device__ __forceinline__ void r2c_tail(float *pl, int lbase, int t, bool active,
+                                         const float2 (&v)[(L / R + T - 1) / T][R], const R2cTail &io)
+{
+    constexpr int NBF = L / R;
+    constexpr int NB = (NBF + T - 1) / T;
+    constexpr bool EXACT = (NBF % T) == 0;
+    constexpr int H = L / 2;
+    constexpr int NUN = (L + 1) / 2;
+    constexpr int NU = (NUN + T - 1) / T;
+    constexpr bool EVEN = (L % 2) == 0;
+    float2 wk[NU], zk[NU], zm[NU], zh = make_float2(0.f, 0.f), wh = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int u = 0; u < NU; u++) {
+        const int kk = t + u * T;
+        if (active && kk < NUN) wk[u] = io.rtw[kk];
+    }
+    if (EVEN && active && t == 0) wh = io.rtw[H];
+#pragma unroll
+    for (int plane = 0; plane < 2; plane++) {
+        __syncthreads();  // earlier readers of the plane are done
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int b = t + q * T;
+            if (active && (EXACT || b < NBF)) {
+                const int k = b % NS;
+                const int hi = b / NS;
+#pragma unroll
+                for (int r = 0; r < R; r++) pl[lbase + lds_idx_write<NS, R>(hi, k, r)] = plane ? v[q][r].y : v[q][r].x;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+            const int kk = t + u * T;
+            if (active && kk < NUN) {
+                const int km = kk == 0 ? 0 : L - kk;
+                const float x0 = pl[lbase + lds_pad(kk)], x1 = pl[lbase + lds_pad(km)];
+                if (plane) {
+                    zk[u].y = x0;
+                    zm[u].y = x1;
+                } else {
+                    zk[u].x = x0;
+                    zm[u].x = x1;
+                }
+            }
+        }
+        if (EVEN && active && t == 0) {
+            const float x = pl[lbase + lds_pad(H)];
+            if (plane)
+                zh.y = x;
+            else
+                zh.x = x;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NU; u++) {
+        const int kk = t + u * T;
+        if (active && kk < NUN) {
+            gstore(io.out + kk, r2c_bin(zk[u], zm[u], wk[u]));
+            gstore(io.out + (L - kk), r2c_bin(zm[u], zk[u], make_float2(-wk[u].x, wk[u].y)));
+        }
+    }
+    if (EVEN && active && t == 0) gstore(io.out + H, r2c_bin(zh, zh, wh));
+}
+
 // C2C transforms with one LDS plane per transform (half the LDS of the two-plane layout, so
 // twice as many workgroups fit on a CU): contiguous rows, or column tiles whose lanes walk adjacent
 // columns. The first pass reads HBM, the last writes HBM through io.store, and every exchange in
@@ -267,7 +361,9 @@ __device__ __forceinline__ void split_chain(float *pl, const float2 *__restrict_
             dft<R, S>(v[q]);
         }
     }
-    if constexpr (sizeof...(Rest) == 0) {
+    if constexpr (sizeof...(Rest) == 0 && Io::kPairTail) {
+        r2c_tail<L, T, NS, R>(pl, lbase, t, active, v, io);
+    } else if constexpr (sizeof...(Rest) == 0) {
 #pragma unroll
         for (int q = 0; q < NB; q++) {
             const int b = t + q * T;
@@ -318,7 +414,7 @@ __device__ __forceinline__ void split_chain(float *pl, const float2 *__restrict_
     }
 }
 
-template <int L, int T, int S, int R0, int... Rest>
+template <int L, int T, int S, bool R2C, int R0, int... Rest>
 __device__ __forceinline__ void split_rows(const StockhamArgs &a, float *pl, int64_t tile)
 {
     constexpr int NBF0 = L / R0;
@@ -345,7 +441,103 @@ __device__ __forceinline__ void split_rows(const StockhamArgs &a, float *pl, int
             for (int r = 0; r < R0; r++) v[q][r] = gload(g + b + r * NBF0);
         }
     }
-    split_chain<L, T, S, 1, R0, Rest...>(pl, a.tw, flp * a.LP, t, active, v,
+    if constexpr (R2C)
+        split_chain<L, T, S, 1, R0, Rest...>(pl, a.tw, flp * a.LP, t, active, v,
+                                             R2cTail{static_cast<float2 *>(a.out) + ob, a.rtw});
+    else
+        split_chain<L, T, S, 1, R0, Rest...>(pl, a.tw, flp * a.LP, t, active, v,
+                                             RowIo{nullptr, static_cast<float2 *>(a.out) + ob});
+}
+
+// C2R rows with one LDS plane: the half spectrum X[0..L] is read from HBM in pair units (unit kk > 0
+// reads X[kk], X[L-kk] and forms P[kk], P[L-kk]; unit 0 forms P[0] with the imaginary parts of X[0]
+// and X[L] dropped; for even L thread 0 also forms the self-paired P[L/2]), exchanged through the
+// plane into the first pass's layout, and the chain writes the L complex (2L real) outputs to HBM.
+template <int L, int T, int S, int R0, int... Rest>
+__device__ __forceinline__ void split_rows_c2r(const StockhamArgs &a, float *pl, int64_t tile)
+{
+    constexpr int NBF0 = L / R0;
+    constexpr int NB0 = (NBF0 + T - 1) / T;
+    constexpr bool EXACT0 = (NBF0 % T) == 0;
+    constexpr int H = L / 2;
+    constexpr int NUN = (L + 1) / 2;
+    constexpr int NU = (NUN + T - 1) / T;
+    constexpr bool EVEN = (L % 2) == 0;
+    const int F = a.F;
+    const int tid = threadIdx.x;
+    const int flp = tid / T;
+    const int t = tid - flp * T;
+    const int64_t f = tile * F + flp;
+    const bool active = f < a.nfft;
+    int64_t ib = 0, ob = 0;
+    if (active) {
+        ib = tile_base((uint32_t)f, a.tin);
+        ob = tile_base((uint32_t)f, a.tout);
+    }
+    const int lbase = flp * a.LP;
+    const float2 *g = static_cast<const float2 *>(a.in) + ib;
+    float2 p1[NU], p2[NU], ph = make_float2(0.f, 0.f);
+    {
+        float2 xk[NU], xm[NU], wk[NU], xh = make_float2(0.f, 0.f), wh = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+            const int kk = t + u * T;
+            if (active && kk < NUN) {
+                xk[u] = gload(g + kk);
+                xm[u] = gload(g + (L - kk));
+                wk[u] = a.rtw[kk];
+            }
+        }
+        if (EVEN && active && t == 0) {
+            xh = gload(g + H);
+            wh = a.rtw[H];
+        }
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+            const int kk = t + u * T;
+            if (active && kk < NUN) {
+                float2 x0 = xk[u], x1 = xm[u];
+                if (kk == 0) {
+                    x0.y = 0.f;
+                    x1.y = 0.f;
+                }
+                p1[u] = c2r_input(x0, x1, wk[u]);
+                p2[u] = c2r_input(x1, x0, make_float2(-wk[u].x, wk[u].y));
+            }
+        }
+        if (EVEN && active && t == 0) ph = c2r_input(xh, xh, wh);
+    }
+    float2 v[NB0][R0];
+#pragma unroll
+    for (int plane = 0; plane < 2; plane++) {
+        if (plane == 1) __syncthreads();  // plane 0's readers are done
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+            const int kk = t + u * T;
+            if (active && kk < NUN) {
+                pl[lbase + lds_pad(kk)] = plane ? p1[u].y : p1[u].x;
+                if (kk != 0) pl[lbase + lds_pad(L - kk)] = plane ? p2[u].y : p2[u].x;
+            }
+        }
+        if (EVEN && active && t == 0) pl[lbase + lds_pad(H)] = plane ? ph.y : ph.x;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NB0; q++) {
+            const int b = t + q * T;
+            if (active && (EXACT0 || b < NBF0)) {
+#pragma unroll
+                for (int r = 0; r < R0; r++) {
+                    const float x = pl[lbase + lds_idx_read<NBF0>(b, r)];
+                    if (plane)
+                        v[q][r].y = x;
+                    else
+                        v[q][r].x = x;
+                }
+            }
+        }
+    }
+    __syncthreads();  // the first pass writes the plane again
+    split_chain<L, T, S, 1, R0, Rest...>(pl, a.tw, lbase, t, active, v,
                                          RowIo{nullptr, static_cast<float2 *>(a.out) + ob});
 }
 
@@ -367,12 +559,25 @@ template <int L, int T, int S, int... Rs>
 __global__ __launch_bounds__(1024) void stockham_rows_kernel(const StockhamArgs a)
 {
     extern __shared__ float lds[];
-    split_rows<L, T, S, Rs...>(a, lds, tile_index(a));
+    split_rows<L, T, S, false, Rs...>(a, lds, tile_index(a));
+}
+
+// Real-transform rows, single LDS plane: MODE 1 = R2C (half-length C2C + pair post-process),
+// MODE 2 = C2R (pair pre-process + half-length C2C).
+template <int L, int T, int S, int MODE, int... Rs>
+__global__ __launch_bounds__(1024) void stockham_real_rows_kernel(const StockhamArgs a)
+{
+    extern __shared__ float lds[];
+    if constexpr (MODE == 1)
+        split_rows<L, T, S, true, Rs...>(a, lds, tile_index(a));
+    else
+        split_rows_c2r<L, T, S, Rs...>(a, lds, tile_index(a));
 }
 
 // Strided transform of a column tile: element j at in[j * es_in] / out[j * es_out], with the
 // four-step twiddle W^{col * j} applied on store when the stage carries one.
 struct ColIo {
+    static constexpr bool kPairTail = false;
     const float2 *__restrict__ in;
     float2 *__restrict__ out;
     int64_t es_in, es_out;
@@ -760,12 +965,7 @@ __device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds,
                 wk[u] = a.rtw[kk];
             }
         }
-        auto bin = [](float2 z, float2 zc, float2 w) {
-            const float2 iw = mul_si<1>(w);
-            const float2 A = make_float2(0.5f * (1.f - iw.x), -0.5f * iw.y);
-            const float2 B = make_float2(0.5f * (1.f + iw.x), 0.5f * iw.y);
-            return cadd(cmul(z, A), cmul(cconj(zc), B));
-        };
+        auto bin = r2c_bin;
         if (EVEN && tid < nf) {
             const float2 zh = make_float2(re[tid * LP + lds_pad(H)], im[tid * LP + lds_pad(H)]);
             dst[out_base[tid] + (int64_t)H * es] = bin(zh, zh, a.rtw[H]);

@@ -115,6 +115,19 @@ def test_r2c_c2r_lengths(torch_cuda, n):
     assert rel <= tol
 
 
+@pytest.mark.parametrize("n,b", [(1024, 7), (2048, 5), (16384, 3), (8192, 1)])
+def test_r2c_c2r_ragged_batch(torch_cuda, n, b):
+    """Batches that leave the last tile of several transforms partly empty."""
+    x = oracle.uniform(n * b, 7000 + n + b).reshape(b, n)
+    X = r2c(x, [n])
+    rel, _ = report(f"r2c n={n} b={b}", X, oracle.r2c(x))
+    assert rel <= TOL
+    Xh = oracle.r2c(x).astype(np.complex64)
+    y = c2r(Xh, [n])
+    rel, _ = report(f"c2r n={n} b={b}", y, oracle.c2r(Xh, n))
+    assert rel <= TOL
+
+
 def test_reference_example_input(torch_cuda):
     """The reference example's input x[i] = (i, i+1) over batch 32 x N 128
     (example/A2/FFT/1d/example_acl_fft_c2c_1d.cpp:99-110), against the committed golden vectors."""
@@ -133,10 +146,11 @@ def test_reference_example_input(torch_cuda):
     assert rel <= 1e-6
 
 
-def test_direction_quirks(torch_cuda):
+@pytest.mark.parametrize("n", [1000, 4096])
+def test_direction_quirks(torch_cuda, n):
     """R2C with INVERSE computes the e^{+} transform; C2R with FORWARD the e^{-} transform of the
     Hermitian extension (dft_r2c_core.cpp:96-101, dft_c2r_core.cpp:90-105)."""
-    n, b = 1000, 4
+    b = 4
     x = oracle.uniform(n * b, 99).reshape(b, n)
     X = r2c(x, [n], INV)
     rel, _ = report("r2c inverse", X, oracle.r2c(x, oracle.INVERSE))
