@@ -483,7 +483,9 @@ struct Builder {
         if (a.col_direct && inst->cols_fwd != nullptr && (L & (L - 1)) == 0 && env_flag("SIP_FFT_SPLIT_COLS", true))
             a.split_lds = 1;
         // Fused real transforms on contiguous rows: single plane as well (pair pre-process read
-        // straight from HBM for C2R, post-process through the plane for R2C).
+        // straight from HBM for C2R, post-process through the plane for R2C). Measured on MI355X:
+        // 8192 x 4096 R2C 58.1 -> 47.5 us, C2R 58.8 -> 46.7 us; 12000 x 2048 77 / 82 -> 68 / 67 us;
+        // 2000 x 8192 40 / 38 -> 36 / 34 us; 3000 and 6000 R2C unchanged.
         const bool real_rows = (io_mode == IO_R2C_EVEN && a.direct_in && a.tout.es == 1 && !a.tout.unit_f) ||
                                (io_mode == IO_C2R_EVEN && a.direct_out && a.tin.es == 1 && !a.tin.unit_f);
         if (real_rows && inst->real_rows[0] != nullptr && env_flag("SIP_FFT_SPLIT_REAL", true)) a.split_lds = 1;

@@ -140,8 +140,9 @@ def instances():
 # longest leaf used for column tiles (the host's kColumnLeafMax): these lengths also get the
 # single-plane column-tile kernel
 COLUMN_LEAF_MAX = 1024
-# half lengths of the real transforms that also get single-plane R2C / C2R row kernels
-REAL_ROWS_LENGTHS = (512, 1024, 2048, 4096, 8192)
+# half lengths (with 32 or more threads per transform) of the real transforms that also get
+# single-plane R2C / C2R row kernels
+REAL_ROWS_MAX = 8192
 
 
 def main():
@@ -187,7 +188,7 @@ def main():
                 cols = f"(const void *)&{cf}, (const void *)&{ci}"
             else:
                 cols = "nullptr, nullptr"
-            if n in REAL_ROWS_LENGTHS:
+            if t >= 32 and n <= REAL_ROWS_MAX:
                 names = [f"stockham_real_rows_kernel<{n}, {t}, {sg}, {m}, {rs}>" for m in (1, 2) for sg in (-1, 1)]
                 for nm in names:
                     lines.append(f"template __global__ void {nm}(const StockhamArgs);")

@@ -104,8 +104,8 @@ struct StockhamInstance {
     const void *rows_inv;
     const void *cols_fwd;    // single-plane column-tile kernels (L <= 1024 only, else null)
     const void *cols_inv;
-    const void *real_rows[4];  // single-plane R2C fwd / inv, C2R fwd / inv rows (power-of-two L
-                               // 512..8192 only, else null)
+    const void *real_rows[4];  // single-plane R2C fwd / inv, C2R fwd / inv rows (L <= 8192 with
+                               // T >= 32 only, else null)
 };
 
 // Registry entry of a fused 2D tile instance (stockham2d.h). The radix sequences, and so the
