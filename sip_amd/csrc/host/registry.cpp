@@ -1,4 +1,5 @@
 // Lookup of the compiled Stockham leaf kernels (generated instance tables).
+#include <cstdlib>
 #include <map>
 
 #include "plan.h"
@@ -35,9 +36,17 @@ const StockhamInstance *find_instance(int64_t L)
 int64_t instance_count() { return (int64_t)instances().size(); }
 
 extern const Stockham2dInstance g_stockham2d[9];
+extern const Stockham2dInstance g_stockham2d_split[1];
 
 const Stockham2dInstance *find_2d_instance(int64_t X, int64_t Y)
 {
+    // 128 x 128 slices take the single-plane tile by default (two workgroups per CU): 3D 128^3 x 64
+    // 1035 -> 983 us, 2D 128^2 x 4096 338 -> 319 us on MI355X; SIP_FFT_TILE2D_SPLIT=0 selects
+    // the two-plane tile
+    if (X == 128 && Y == 128) {
+        const char *e = std::getenv("SIP_FFT_TILE2D_SPLIT");
+        if (!e || std::atoi(e) != 0) return &g_stockham2d_split[0];
+    }
     for (const Stockham2dInstance &t : g_stockham2d)
         if (t.X == X && t.Y == Y) return &t;
     return nullptr;

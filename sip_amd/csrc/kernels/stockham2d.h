@@ -97,6 +97,175 @@ struct Tile2d<X, TX, Y, TY, S, Radices<RX...>, Radices<RY...>> {
     }
 };
 
+template <int A, int... Rest>
+constexpr int last_of()
+{
+    if constexpr (sizeof...(Rest) == 0)
+        return A;
+    else
+        return last_of<Rest...>();
+}
+
+// The same slice with one LDS plane (half the LDS, so that a 128 x 128 slice leaves room for a
+// second workgroup on the CU): every hand-over (HBM order -> row passes -> columns -> HBM order)
+// moves the real parts through the plane, then the imaginary parts, and the radix passes of each
+// axis are the single-plane chains of the 1D row kernel.
+template <int X, int TX, int Y, int TY, int S, class RX, class RY>
+struct Tile2dSplit;
+
+template <int X, int TX, int Y, int TY, int S, int... RX, int... RY>
+struct Tile2dSplit<X, TX, Y, TY, S, Radices<RX...>, Radices<RY...>> {
+    static constexpr int kThreads = (X * TY > Y * TX) ? X * TY : Y * TX;
+    static constexpr int kLpy = Y + Y / 16 + 1;  // padded row pitch, [x][y] layout (odd)
+    static constexpr int kLpx = X + X / 16 + 1;  // padded column pitch, [y][x] layout (odd)
+    static constexpr int kPlane = (X * kLpy > Y * kLpx) ? X * kLpy : Y * kLpx;
+    static constexpr int kPer = (X * Y + kThreads - 1) / kThreads;  // elements per thread
+    // first / last radix of each axis and the register shapes of those passes
+    static constexpr int RY0 = first_of<RY...>(), RYL = last_of<RY...>();
+    static constexpr int RX0 = first_of<RX...>(), RXL = last_of<RX...>();
+    static constexpr int NBY0 = (Y / RY0 + TY - 1) / TY, NBYL = (Y / RYL + TY - 1) / TY;
+    static constexpr int NBX0 = (X / RX0 + TX - 1) / TX, NBXL = (X / RXL + TX - 1) / TX;
+    static constexpr int NSY = Y / RYL, NSX = X / RXL;  // NS of the last pass of each axis
+
+    __device__ __forceinline__ static void run(const StockhamArgs &a, float *pl)
+    {
+        const int tid = threadIdx.x;
+        const int64_t slice = blockIdx.x;
+        const float2 *src = static_cast<const float2 *>(a.in) + slice * (int64_t)(X * Y);
+        float2 *dst = static_cast<float2 *>(a.out) + slice * (int64_t)(X * Y);
+        // row view: transform xr (length Y), thread tr; column view: transform yc (length X), tc
+        const int xr = tid / TY, tr = tid - xr * TY;
+        const bool ract = xr < X;
+        const int yc = tid / TX, tc = tid - yc * TX;
+        const bool cact = yc < Y;
+        const int rbase = (ract ? xr : 0) * kLpy, cbase = (cact ? yc : 0) * kLpx;
+
+        // HBM [x][y] -> first row pass: consecutive lanes read consecutive elements
+        float2 rin[NBY0][RY0];
+        {
+            float2 v[kPer];
+#pragma unroll
+            for (int u = 0; u < kPer; u++) {
+                const int i = tid + u * kThreads;
+                if (i < X * Y) v[u] = src[i];
+            }
+#pragma unroll
+            for (int plane = 0; plane < 2; plane++) {
+                if (plane) __syncthreads();
+#pragma unroll
+                for (int u = 0; u < kPer; u++) {
+                    const int i = tid + u * kThreads;
+                    if (i < X * Y) {
+                        const int x = i / Y, y = i - x * Y;
+                        pl[x * kLpy + lds_pad(y)] = plane ? v[u].y : v[u].x;
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < NBY0; q++) {
+                    const int b = tr + q * TY;
+                    if (ract && b < Y / RY0) {
+#pragma unroll
+                        for (int r = 0; r < RY0; r++) {
+                            const float x = pl[rbase + lds_idx_read<Y / RY0>(b, r)];
+                            if (plane)
+                                rin[q][r].y = x;
+                            else
+                                rin[q][r].x = x;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        // row passes
+        float2 rout[NBYL][RYL];
+        split_chain<Y, TY, S, 1, RY...>(pl, a.tw, rbase, tr, ract, rin, RegTail<RYL>{rout});
+        // row outputs (x = xr, y = j) -> [y][x] -> first column pass
+        float2 cin[NBX0][RX0];
+#pragma unroll
+        for (int plane = 0; plane < 2; plane++) {
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < NBYL; q++) {
+                const int b = tr + q * TY;
+                if (ract && b < Y / RYL) {
+                    const int k = b % NSY;
+                    const int d = (b / NSY) * NSY * RYL + k;
+#pragma unroll
+                    for (int r = 0; r < RYL; r++) {
+                        const int y = d + r * NSY;
+                        pl[y * kLpx + lds_pad(xr)] = plane ? rout[q][r].y : rout[q][r].x;
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < NBX0; q++) {
+                const int b = tc + q * TX;
+                if (cact && b < X / RX0) {
+#pragma unroll
+                    for (int r = 0; r < RX0; r++) {
+                        const float x = pl[cbase + lds_idx_read<X / RX0>(b, r)];
+                        if (plane)
+                            cin[q][r].y = x;
+                        else
+                            cin[q][r].x = x;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // column passes
+        float2 cout[NBXL][RXL];
+        split_chain<X, TX, S, 1, RX...>(pl, a.tw2, cbase, tc, cact, cin, RegTail<RXL>{cout});
+        // column outputs (y = yc, x = j) -> [x][y] -> HBM in order
+        float2 v[kPer];
+#pragma unroll
+        for (int plane = 0; plane < 2; plane++) {
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < NBXL; q++) {
+                const int b = tc + q * TX;
+                if (cact && b < X / RXL) {
+                    const int k = b % NSX;
+                    const int d = (b / NSX) * NSX * RXL + k;
+#pragma unroll
+                    for (int r = 0; r < RXL; r++) {
+                        const int x = d + r * NSX;
+                        pl[x * kLpy + lds_pad(yc)] = plane ? cout[q][r].y : cout[q][r].x;
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < kPer; u++) {
+                const int i = tid + u * kThreads;
+                if (i < X * Y) {
+                    const int x = i / Y, y = i - x * Y;
+                    const float val = pl[x * kLpy + lds_pad(y)];
+                    if (plane)
+                        v[u].y = val;
+                    else
+                        v[u].x = val;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            const int i = tid + u * kThreads;
+            if (i < X * Y) dst[i] = v[u];
+        }
+    }
+};
+
+template <int X, int TX, int Y, int TY, int S, class RX, class RY>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void stockham2d_split_kernel(const StockhamArgs a)
+{
+    extern __shared__ float lds[];
+    Tile2dSplit<X, TX, Y, TY, S, RX, RY>::run(a, lds);
+}
+
 template <int X, int TX, int Y, int TY, int S, class RX, class RY>
 __global__ __launch_bounds__(1024) void stockham2d_kernel(const StockhamArgs a)
 {

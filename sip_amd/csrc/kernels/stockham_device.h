@@ -136,6 +136,7 @@ __device__ __forceinline__ void tw_expand(float2 (&w)[R])
 // HBM endpoints of the direct passes: element j of this thread's transform.
 struct RowIo {  // contiguous transform
     static constexpr bool kPairTail = false;
+    static constexpr bool kRegTail = false;
     const float2 *__restrict__ in;
     float2 *__restrict__ out;
     __device__ __forceinline__ float2 load(int j) const { return gload(in + j); }
@@ -265,6 +266,7 @@ __device__ __forceinline__ float2 c2r_input(float2 x, float2 xc, float2 w)
 // bins 0 and L from Z[0]; for even L thread 0 also writes the self-paired bin L/2).
 struct R2cTail {
     static constexpr bool kPairTail = true;
+    static constexpr bool kRegTail = false;
     float2 *__restrict__ out;  // this transform's L + 1 bins
     const float2 *__restrict__ rtw;
 };
@@ -335,6 +337,16 @@ __device__ __forceinline__ void r2c_tail(float *pl, int lbase, int t, bool activ
     if (EVEN && active && t == 0) gstore(io.out + H, r2c_bin(zh, zh, wh));
 }
 
+// End of a single-plane chain that keeps the last pass's outputs in registers: o[q][r] holds
+// element d + r NS of the transform (d = (b / NS) NS R + b mod NS, b = t + q T), for the caller to
+// move on (the fused 2D tile's transposes).
+template <int R>
+struct RegTail {
+    static constexpr bool kPairTail = false;
+    static constexpr bool kRegTail = true;
+    float2 (*o)[R];
+};
+
 // C2C transforms with one LDS plane per transform (half the LDS of the two-plane layout, so
 // twice as many workgroups fit on a CU): contiguous rows, or column tiles whose lanes walk adjacent
 // columns. The first pass reads HBM, the last writes HBM through io.store, and every exchange in
@@ -363,6 +375,11 @@ __device__ __forceinline__ void split_chain(float *pl, const float2 *__restrict_
     }
     if constexpr (sizeof...(Rest) == 0 && Io::kPairTail) {
         r2c_tail<L, T, NS, R>(pl, lbase, t, active, v, io);
+    } else if constexpr (sizeof...(Rest) == 0 && Io::kRegTail) {
+#pragma unroll
+        for (int q = 0; q < NB; q++)
+#pragma unroll
+            for (int r = 0; r < R; r++) io.o[q][r] = v[q][r];
     } else if constexpr (sizeof...(Rest) == 0) {
 #pragma unroll
         for (int q = 0; q < NB; q++) {
@@ -578,6 +595,7 @@ __global__ __launch_bounds__(1024) void stockham_real_rows_kernel(const Stockham
 // four-step twiddle W^{col * j} applied on store when the stage carries one.
 struct ColIo {
     static constexpr bool kPairTail = false;
+    static constexpr bool kRegTail = false;
     const float2 *__restrict__ in;
     float2 *__restrict__ out;
     int64_t es_in, es_out;

@@ -131,3 +131,21 @@ def test_fused_2d_tiles(torch_cuda, x, y):
     got = c2c(a3, [12, x, y], FWD)
     rel, _ = report(f"3d 12x{x}x{y} (fused last two axes)", got, oracle.c2c_nd(a3, 3))
     assert rel <= 2e-6
+
+
+@pytest.mark.parametrize("split", ["0", "1"])
+def test_fused_2d_128_planes(torch_cuda, monkeypatch, split):
+    """The 128 x 128 slice with two LDS planes and with one (SIP_FFT_TILE2D_SPLIT), both
+    directions, a batch that is not a multiple of anything, and a 3D transform."""
+    monkeypatch.setenv("SIP_FFT_TILE2D_SPLIT", split)
+    b = 7
+    a = oracle.uniform_complex((b, 128, 128), 0x5EED0500 + int(split))
+    for direction in (FWD, INV):
+        got = c2c(a, [128, 128], direction)
+        rel, _ = report(f"fused 2d 128x128 split={split} dir={direction:#x}", got,
+                        oracle.c2c_nd(a, 2, sign_of(direction)))
+        assert rel <= 2e-6
+    a3 = oracle.uniform_complex((2, 6, 128, 128), 0x5EED0600 + int(split))
+    got = c2c(a3, [6, 128, 128], INV)
+    rel, _ = report(f"3d 6x128x128 split={split}", got, oracle.c2c_nd(a3, 3, sign_of(INV)))
+    assert rel <= 2e-6
