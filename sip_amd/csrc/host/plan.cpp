@@ -413,7 +413,12 @@ struct Builder {
             if (want <= 0) {
                 if (const char *e = std::getenv("SIP_FFT_COL_F")) want = std::atoi(e);
             }
-            F = want > 0 ? want : std::min(64, std::max(16, (512 / T) & ~15));
+            // Mixed lengths: 16 wide (whole lines, small LDS). Measured on MI355X, four-step
+            // C2C rows: 43200 x 1056 459 -> 401 us, 36000 x 864 350 -> 277 us, 27000 x 604
+            // 168 -> 154 us; power-of-two columns keep the wider tiles (the 256-row stage B of
+            // 64000 x 635: 360 us at 32 wide, 393 us at 16).
+            const bool pow2 = (L & (L - 1)) == 0;
+            F = want > 0 ? want : (pow2 ? std::min(64, std::max(16, (512 / T) & ~15)) : 16);
             if (F >= 16) F &= ~15;
             while (F > 1 && ((size_t)F * per > 160 * 1024 || F * T > 1024)) F /= 2;
         } else {
