@@ -193,11 +193,30 @@ class FftHandle:
     """One asdFftHandle: Create -> MakePlan -> workspace -> SetStream -> Exec -> Synchronize -> Destroy,
     the call order of the reference's examples (example/A2/FFT/1d/example_acl_fft_c2c_1d.cpp:121-145)."""
 
+    _TENSOR_CACHE = 16  # descriptors kept per handle (oldest dropped first)
+
     def __init__(self):
         h = ctypes.c_void_p()
         _check(lib().asdFftCreate(ctypes.byref(h)), "asdFftCreate")
         self.h = h
         self._ws = None
+        self._tensors = {}  # (address, shape, dtype) -> AclTensor
+
+    def _desc(self, t):
+        """aclTensor descriptor of a contiguous tensor, reused across execs on the same buffer (a
+        descriptor holds only address, shape and dtype, as a caller of the C API would keep it)."""
+        import torch
+        if not t.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        key = (t.data_ptr(), tuple(t.shape), t.dtype)
+        d = self._tensors.get(key)
+        if d is None:
+            if t.dtype not in (torch.complex64, torch.float32):
+                raise ValueError(f"unsupported dtype {t.dtype}")
+            if len(self._tensors) >= self._TENSOR_CACHE:
+                self._tensors.pop(next(iter(self._tensors))).close()
+            d = self._tensors[key] = AclTensor.from_torch(t)
+        return d
 
     # -- planning
     def make_plan_1d(self, n, fft_type, direction, batch, dim_type=ASCEND_FFT_HORIZONTAL):
@@ -237,12 +256,7 @@ class FftHandle:
 
     # -- execution on torch tensors (no copies; descriptors only)
     def _exec(self, fn, x, y):
-        tx, ty = AclTensor.from_torch(x), AclTensor.from_torch(y)
-        try:
-            return fn(self.h, ctypes.c_void_p(tx.ptr), ctypes.c_void_p(ty.ptr))
-        finally:
-            tx.close()
-            ty.close()
+        return fn(self.h, ctypes.c_void_p(self._desc(x).ptr), ctypes.c_void_p(self._desc(y).ptr))
 
     def exec_c2c(self, x, y):
         return self._exec(lib().asdFftExecC2C, x, y)
@@ -297,6 +311,9 @@ class FftHandle:
             st = lib().asdFftDestroy(self.h)
             self.h = None
             self._ws = None
+            for d in self._tensors.values():
+                d.close()
+            self._tensors = {}
             return st
         return ACL_SUCCESS
 
