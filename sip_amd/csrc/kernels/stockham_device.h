@@ -144,7 +144,7 @@ struct RowIo {  // contiguous transform
 };
 
 // One radix-R pass over a transform of length L held in LDS, T threads per transform,
-// NS = product of the radices of the earlier passes, TW = this pass's twiddles [r-1][k].
+// NS = product of the radices of the earlier passes, TW = this pass's twiddle slots [k][tw_rows<R>].
 // GIN: the pass reads its inputs straight from HBM (first pass: element b + r*L/R, through
 // io.load) instead of LDS. GOUT: the pass writes its outputs straight to HBM (last pass: the
 // autosort output index is b + r*L/R as well, through io.store).
@@ -773,8 +773,7 @@ __device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds,
         }
         if (EVEN && tid < nf) {
             const float2 xh = src[in_base[tid] + (int64_t)H * es];
-            const float2 c = cconj(xh);
-            const float2 p = cadd(cadd(xh, c), mul_si<1>(cmul(a.rtw[H], csub(xh, c))));
+            const float2 p = c2r_input(xh, xh, a.rtw[H]);
             re[tid * LP + lds_pad(H)] = p.x;
             im[tid * LP + lds_pad(H)] = p.y;
         }
@@ -791,15 +790,12 @@ __device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds,
                 a1.y = 0.f;
             }
             {
-                const float2 c = cconj(a1);
-                const float2 p = cadd(cadd(a0, c), mul_si<1>(cmul(wk[u], csub(a0, c))));
+                const float2 p = c2r_input(a0, a1, wk[u]);
                 pr[lds_pad(kk)] = p.x;
                 pi[lds_pad(kk)] = p.y;
             }
             if (kk != 0) {
-                const float2 w2 = make_float2(-wk[u].x, wk[u].y);
-                const float2 c = cconj(a0);
-                const float2 p = cadd(cadd(a1, c), mul_si<1>(cmul(w2, csub(a1, c))));
+                const float2 p = c2r_input(a1, a0, make_float2(-wk[u].x, wk[u].y));
                 pr[lds_pad(L - kk)] = p.x;
                 pi[lds_pad(L - kk)] = p.y;
             }
@@ -983,10 +979,9 @@ __device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds,
                 wk[u] = a.rtw[kk];
             }
         }
-        auto bin = r2c_bin;
         if (EVEN && tid < nf) {
             const float2 zh = make_float2(re[tid * LP + lds_pad(H)], im[tid * LP + lds_pad(H)]);
-            dst[out_base[tid] + (int64_t)H * es] = bin(zh, zh, a.rtw[H]);
+            dst[out_base[tid] + (int64_t)H * es] = r2c_bin(zh, zh, a.rtw[H]);
         }
 #pragma unroll
         for (int u = 0; u < NU; u++) {
@@ -996,8 +991,8 @@ __device__ __forceinline__ void stockham_body(const StockhamArgs &a, float *lds,
             if (i >= F * NUN || fl >= nf) continue;
             float2 *o = dst + out_base[fl];
             const float2 w2 = make_float2(-wk[u].x, wk[u].y);  // w^{L - kk}
-            o[(int64_t)kk * es] = bin(zk[u], zm[u], wk[u]);
-            o[(int64_t)(L - kk) * es] = bin(zm[u], zk[u], w2);
+            o[(int64_t)kk * es] = r2c_bin(zk[u], zm[u], wk[u]);
+            o[(int64_t)(L - kk) * es] = r2c_bin(zm[u], zk[u], w2);
         }
     } else if (a.io_mode == IO_C2R_FULL) {
         float *dst = static_cast<float *>(a.out);
