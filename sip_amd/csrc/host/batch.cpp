@@ -195,19 +195,25 @@ std::vector<Tune> candidates(int64_t n)
             c.push_back(t);
         }
     } else {
+        // four-step splits n = a * b: the 8 most balanced ones (the default plan takes the most
+        // balanced pair of column leaves), each with three column-tile widths
+        std::vector<std::pair<double, int64_t>> splits;
         for (int64_t a = 2; a <= 1024 && a < n; a++) {
             if (n % a || !find_instance(a)) continue;
             const int64_t b = n / a;
             if (!find_instance(b) && b > 16384) continue;
             if (a * 16 < b || b * 16 < a) continue;  // keep splits within 16x of balanced
-            for (int cf : {0, 16, 64}) {
+            splits.emplace_back(std::fabs(std::log((double)a) - std::log((double)b)), a);
+        }
+        std::sort(splits.begin(), splits.end());
+        if (splits.size() > 8) splits.resize(8);
+        for (const auto &sp : splits)
+            for (int cf : {0, 16, 32}) {
                 Tune t;
-                t.split_l1 = a;
+                t.split_l1 = sp.second;
                 t.col_F = cf;
                 c.push_back(t);
             }
-            if (c.size() > 24) break;
-        }
     }
     return c;
 }
