@@ -141,6 +141,13 @@ struct RowIo {  // contiguous transform
     float2 *__restrict__ out;
     __device__ __forceinline__ float2 load(int j) const { return gload(in + j); }
     __device__ __forceinline__ void store(int j, float2 v) const { gstore(out + j, v); }
+    // the R outputs d + r NS of one butterfly
+    template <int R, int NS>
+    __device__ __forceinline__ void store_bf(int d, const float2 (&v)[R]) const
+    {
+#pragma unroll
+        for (int r = 0; r < R; r++) gstore(out + d + r * NS, v[r]);
+    }
 };
 
 // One radix-R pass over a transform of length L held in LDS, T threads per transform,
@@ -187,11 +194,11 @@ __device__ __forceinline__ void pass_static(float *re, float *im, const float2 *
             const int k = b % NS;
             const int hi = b / NS;
             const int d = hi * NS * R + k;
+            if constexpr (GOUT) {
+                io.template store_bf<R, NS>(d, v[q]);
+            } else {
 #pragma unroll
-            for (int r = 0; r < R; r++) {
-                if constexpr (GOUT) {
-                    io.store(d + r * NS, v[q][r]);
-                } else {
+                for (int r = 0; r < R; r++) {
                     const int j = lds_idx_write<NS, R>(hi, k, r);
                     re[lbase + j] = v[q][r].x;
                     im[lbase + j] = v[q][r].y;
@@ -387,8 +394,7 @@ __device__ __forceinline__ void split_chain(float *pl, const float2 *__restrict_
             if (active && (EXACT || b < NBF)) {
                 const int k = b % NS;
                 const int d = (b / NS) * NS * R + k;
-#pragma unroll
-                for (int r = 0; r < R; r++) io.store(d + r * NS, v[q][r]);
+                io.template store_bf<R, NS>(d, v[q]);
             }
         }
     } else {
@@ -606,6 +612,29 @@ struct ColIo {
     {
         if (a->tw4_ntot != 0) v = cmul(v, tw4_root(*a, col * (int64_t)j));
         gstore(out + (int64_t)j * es_out, v);
+    }
+    // The R outputs d + r NS of one butterfly. With a four-step twiddle, W^{col (d + r NS)} =
+    // W^{col d} (W^{col NS})^r: W^{col d} and the powers-of-two r of W^{col NS} come from the
+    // two-level table (every argument is below col * L < ntot), the other r are multiplied
+    // together as for the pass twiddles, so a radix-16 butterfly loads 10 table entries instead
+    // of 32.
+    template <int R, int NS>
+    __device__ __forceinline__ void store_bf(int d, const float2 (&v)[R]) const
+    {
+        if (a->tw4_ntot == 0) {
+#pragma unroll
+            for (int r = 0; r < R; r++) gstore(out + (int64_t)(d + r * NS) * es_out, v[r]);
+            return;
+        }
+        const int64_t step = col * (int64_t)NS;
+        float2 w[R];
+#pragma unroll
+        for (int i = 0; (1 << i) < R; i++) w[1 << i] = tw4_root(*a, step << i);
+        const float2 base = tw4_root(*a, col * (int64_t)d);
+        tw_expand<R>(w);
+        gstore(out + (int64_t)d * es_out, cmul(v[0], base));
+#pragma unroll
+        for (int r = 1; r < R; r++) gstore(out + (int64_t)(d + r * NS) * es_out, cmul(v[r], cmul(base, w[r])));
     }
 };
 
