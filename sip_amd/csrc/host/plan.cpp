@@ -480,13 +480,18 @@ struct Builder {
         // available as the tuning switch SIP_FFT_XCD=1.
         a.split_lds = rows && (p.tune.split_lds >= 0 ? p.tune.split_lds == 1
                                                      : env_flag("SIP_FFT_SPLIT_LDS", T >= 32));
-        // Column tiles with direct HBM passes of power-of-two length: single plane too (a 1024 x 16
-        // tile needs 70 KiB instead of 139 KiB, so two workgroups fit on a CU). Measured on
-        // MI355X: 2D 1024^2 x 128 892 -> 863 us, 1024^2 x 8 52 -> 48 us, 512^2 x 256 445 -> 422 us,
-        // 2^20 x 64 606 -> 566 us; 1000^2 x 64 lost (482 -> 506 us), so mixed lengths keep two
-        // planes.
-        if (a.col_direct && inst->cols_fwd != nullptr && (L & (L - 1)) == 0 && env_flag("SIP_FFT_SPLIT_COLS", true))
-            a.split_lds = 1;
+        // Column tiles with direct HBM passes: single plane too (a 1024 x 16 tile needs 70 KiB
+        // instead of 139 KiB, so two workgroups fit on a CU). Measured on MI355X: 2D 1024^2 x 128
+        // 892 -> 863 us, 1024^2 x 8 52 -> 48 us, 512^2 x 256 445 -> 422 us, 2^20 x 64 606 -> 566 us;
+        // with 16-wide mixed-length tiles and the derived four-step twiddles, mixed lengths gain
+        // too: four-step 43200 x 1056 387-399 -> 346 us, 64000 x 635 370 -> 337 us, 2D 1000^2 x 64
+        // 528-535 -> 500 us, 3D 240^3 x 8 1155 -> 1086 us.
+        {
+            const char *e = std::getenv("SIP_FFT_SPLIT_COLS");
+            const int mode = e ? std::atoi(e) : 2;  // 0 never, 1 power-of-two lengths, 2 every length
+            if (a.col_direct && inst->cols[0] != nullptr && (mode == 2 || (mode == 1 && (L & (L - 1)) == 0)))
+                a.split_lds = 1;
+        }
         // Fused real transforms on contiguous rows: single plane as well (pair pre-process read
         // straight from HBM for C2R, post-process through the plane for R2C). Measured on MI355X:
         // 8192 x 4096 R2C 58.1 -> 47.5 us, C2R 58.8 -> 46.7 us; 12000 x 2048 77 / 82 -> 68 / 67 us;
@@ -959,7 +964,7 @@ Plan::~Plan()
 static const void *stage_kernel(const Stage &s)
 {
     if (s.inst2d) return s.sign < 0 ? s.inst2d->fwd : s.inst2d->inv;
-    if (s.sa.split_lds && s.sa.col_direct) return s.sign < 0 ? s.inst->cols_fwd : s.inst->cols_inv;
+    if (s.sa.split_lds && s.sa.col_direct) return s.inst->cols[(s.sa.tw4_ntot != 0 ? 2 : 0) + (s.sign < 0 ? 0 : 1)];
     if (s.sa.split_lds && s.sa.io_mode == IO_R2C_EVEN) return s.inst->real_rows[s.sign < 0 ? 0 : 1];
     if (s.sa.split_lds && s.sa.io_mode == IO_C2R_EVEN) return s.inst->real_rows[s.sign < 0 ? 2 : 3];
     if (s.sa.split_lds) return s.sign < 0 ? s.inst->rows_fwd : s.inst->rows_inv;

@@ -181,13 +181,13 @@ def main():
             else:
                 rows = "nullptr, nullptr"
             if n <= COLUMN_LEAF_MAX:
-                cf = f"stockham_cols_kernel<{n}, {t}, -1, {rs}>"
-                ci = f"stockham_cols_kernel<{n}, {t}, 1, {rs}>"
-                lines.append(f"template __global__ void {cf}(const StockhamArgs);")
-                lines.append(f"template __global__ void {ci}(const StockhamArgs);")
-                cols = f"(const void *)&{cf}, (const void *)&{ci}"
+                names = [f"stockham_cols_kernel<{n}, {t}, {sg}, {tw4}, {rs}>" for tw4 in ("false", "true")
+                         for sg in (-1, 1)]
+                for nm in names:
+                    lines.append(f"template __global__ void {nm}(const StockhamArgs);")
+                cols = "{" + ", ".join(f"(const void *)&{nm}" for nm in names) + "}"
             else:
-                cols = "nullptr, nullptr"
+                cols = "{nullptr, nullptr, nullptr, nullptr}"
             if t >= 32 and n <= REAL_ROWS_MAX:
                 names = [f"stockham_real_rows_kernel<{n}, {t}, {sg}, {m}, {rs}>" for m in (1, 2) for sg in (-1, 1)]
                 for nm in names:

@@ -10,7 +10,7 @@ constexpr int count_args(A...) { return (int)sizeof...(A); }
 #define SIP_MF(L, DT, NT, ...)                                                                     \
     {L, L / 16, count_args(__VA_ARGS__), {__VA_ARGS__},                         \
      (const void *)&stockham_mfma_kernel<L, L / 16, -1, DT, NT, __VA_ARGS__>,                       \
-     (const void *)&stockham_mfma_kernel<L, L / 16, 1, DT, NT, __VA_ARGS__>, nullptr, nullptr, nullptr, nullptr, {nullptr, nullptr, nullptr, nullptr}}
+     (const void *)&stockham_mfma_kernel<L, L / 16, 1, DT, NT, __VA_ARGS__>, nullptr, nullptr, {nullptr, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}}
 
 #define SIP_MF_SET(DT, NT)                                                                         \
     {SIP_MF(128, DT, NT, 16, 8), SIP_MF(256, DT, NT, 16, 16), SIP_MF(512, DT, NT, 16, 16, 2),       \

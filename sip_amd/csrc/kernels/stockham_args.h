@@ -102,8 +102,8 @@ struct StockhamInstance {
     const void *inv;
     const void *rows_fwd;    // single-plane contiguous-row kernels (T >= 16 only, else null)
     const void *rows_inv;
-    const void *cols_fwd;    // single-plane column-tile kernels (L <= 1024 only, else null)
-    const void *cols_inv;
+    const void *cols[4];     // single-plane column-tile kernels, fwd / inv without and fwd / inv
+                             // with the four-step twiddle (L <= 1024 only, else null)
     const void *real_rows[4];  // single-plane R2C fwd / inv, C2R fwd / inv rows (L <= 8192 with
                                // T >= 32 only, else null)
 };

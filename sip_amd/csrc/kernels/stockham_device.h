@@ -598,8 +598,12 @@ __global__ __launch_bounds__(1024) void stockham_real_rows_kernel(const Stockham
 }
 
 // Strided transform of a column tile: element j at in[j * es_in] / out[j * es_out], with the
-// four-step twiddle W^{col * j} applied on store when the stage carries one.
-struct ColIo {
+// four-step twiddle W^{col * j} applied on store when the stage carries one. TW: 0 = the stage has
+// no four-step twiddle, 1 = it has one, 2 = decided at run time (tw4_ntot). The single-plane
+// column kernels are instantiated for 0 and 1 so that the twiddle code does not raise the register
+// budget of the plain column passes (cols 1024: 61 VGPRs without, 76 with).
+template <int TW>
+struct ColIoT {
     static constexpr bool kPairTail = false;
     static constexpr bool kRegTail = false;
     const float2 *__restrict__ in;
@@ -610,7 +614,7 @@ struct ColIo {
     __device__ __forceinline__ float2 load(int j) const { return gload(in + (int64_t)j * es_in); }
     __device__ __forceinline__ void store(int j, float2 v) const
     {
-        if (a->tw4_ntot != 0) v = cmul(v, tw4_root(*a, col * (int64_t)j));
+        if (TW == 1 || (TW == 2 && a->tw4_ntot != 0)) v = cmul(v, tw4_root(*a, col * (int64_t)j));
         gstore(out + (int64_t)j * es_out, v);
     }
     // The R outputs d + r NS of one butterfly. With a four-step twiddle, W^{col (d + r NS)} =
@@ -621,7 +625,7 @@ struct ColIo {
     template <int R, int NS>
     __device__ __forceinline__ void store_bf(int d, const float2 (&v)[R]) const
     {
-        if (a->tw4_ntot == 0) {
+        if (TW == 0 || (TW == 2 && a->tw4_ntot == 0)) {
 #pragma unroll
             for (int r = 0; r < R; r++) gstore(out + (int64_t)(d + r * NS) * es_out, v[r]);
             return;
@@ -637,11 +641,12 @@ struct ColIo {
         for (int r = 1; r < R; r++) gstore(out + (int64_t)(d + r * NS) * es_out, cmul(v[r], cmul(base, w[r])));
     }
 };
+using ColIo = ColIoT<2>;
 
 // Column tile with one LDS plane per transform: lane tid walks transform tid mod F (adjacent
 // columns, so a wave's HBM accesses cover whole lines) and butterflies tid / F; the first pass
 // reads and the last pass writes HBM through ColIo (with the four-step twiddle on store).
-template <int L, int T, int S, int R0, int... Rest>
+template <int L, int T, int S, bool TW4, int R0, int... Rest>
 __device__ __forceinline__ void split_cols(const StockhamArgs &a, float *pl, int64_t tile)
 {
     constexpr int NBF0 = L / R0;
@@ -660,8 +665,8 @@ __device__ __forceinline__ void split_cols(const StockhamArgs &a, float *pl, int
         col = fdiv(f0, a.div_col);
         ob = tile_base((uint32_t)f, a.tout);
     }
-    const ColIo io{static_cast<const float2 *>(a.in) + ib, static_cast<float2 *>(a.out) + ob, a.tin.es, a.tout.es,
-                   col, &a};
+    const ColIoT<TW4 ? 1 : 0> io{static_cast<const float2 *>(a.in) + ib, static_cast<float2 *>(a.out) + ob,
+                                 a.tin.es, a.tout.es, col, &a};
     float2 v[NB0][R0];
 #pragma unroll
     for (int q = 0; q < NB0; q++) {
@@ -674,12 +679,13 @@ __device__ __forceinline__ void split_cols(const StockhamArgs &a, float *pl, int
     split_chain<L, T, S, 1, R0, Rest...>(pl, a.tw, flp * a.LP, t, active, v, io);
 }
 
-// Column tiles, single LDS plane (separate kernel so that its register budget is its own).
-template <int L, int T, int S, int... Rs>
+// Column tiles, single LDS plane (separate kernel so that its register budget is its own); TW4:
+// the stage applies the four-step twiddle on store.
+template <int L, int T, int S, bool TW4, int... Rs>
 __global__ __launch_bounds__(1024) void stockham_cols_kernel(const StockhamArgs a)
 {
     extern __shared__ float lds[];
-    split_cols<L, T, S, Rs...>(a, lds, tile_index(a));
+    split_cols<L, T, S, TW4, Rs...>(a, lds, tile_index(a));
 }
 
 // Pass runners: the radix chain the tile goes through between load and store.
