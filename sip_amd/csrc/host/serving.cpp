@@ -142,6 +142,7 @@ public:
             (void)hipEventSynchronize(l->ev);
             (void)hipEventDestroy(l->ev);
         }
+        for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
         for (hipStream_t s : streams_) (void)hipStreamDestroy(s);
         if (pin_) (void)hipHostFree(pin_);
         if (arena_base_) (void)hipFree(arena_base_);
@@ -260,10 +261,18 @@ private:
     bool retire_locked()
     {
         bool any = false;
+        // launches on one stream complete in order: after the first unfinished one of a stream the
+        // later ones of that stream are not queried
+        std::vector<char> blocked((size_t)nstreams_, 0);
         for (size_t i = 0; i < inflight_.size();) {
             Launch &l = *inflight_[i];
+            if (blocked[(size_t)l.stream]) {
+                i++;
+                continue;
+            }
             const hipError_t q = hipEventQuery(l.ev);
             if (q == hipErrorNotReady) {
+                blocked[(size_t)l.stream] = 1;
                 i++;
                 continue;
             }
@@ -282,7 +291,7 @@ private:
             if (l.tab_off != SIZE_MAX) arena_->release(l.tab_off, l.tab_bytes);
             pin_release(l.pin_off, l.pin_bytes);
             inflight_per_stream_[(size_t)l.stream]--;
-            (void)hipEventDestroy(l.ev);
+            ev_pool_.push_back(l.ev);  // reused by a later launch
             inflight_.erase(inflight_.begin() + (std::ptrdiff_t)i);
             any = true;
         }
@@ -459,7 +468,12 @@ private:
             plan_execute(*plan, nullptr, nullptr);
         else
             plan_execute(*plan, h->in, h->out);
-        SIP_HIP_CHECK(hipEventCreateWithFlags(&l->ev, hipEventDisableTiming));
+        if (!ev_pool_.empty()) {
+            l->ev = ev_pool_.back();
+            ev_pool_.pop_back();
+        } else {
+            SIP_HIP_CHECK(hipEventCreateWithFlags(&l->ev, hipEventDisableTiming));
+        }
         SIP_HIP_CHECK(hipEventRecord(l->ev, streams_[(size_t)sidx]));
         l->reqs = group;
         l->plan = std::move(plan);
@@ -507,6 +521,7 @@ private:
     double last_done_ = 0;
     std::mutex mu_;
     std::condition_variable cv_, done_cv_;
+    std::vector<hipEvent_t> ev_pool_;  // completed launches' events, recorded again by later launches
     bool stop_ = false;
     std::thread worker_;
 };
