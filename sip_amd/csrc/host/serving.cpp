@@ -389,7 +389,9 @@ private:
         }
         auto plan = std::make_unique<Plan>();
         plan->direction = kDirForward;
-        const bool gathered = gathered_supported(type, n);
+        // one request: its rows are contiguous, so the plain plan runs it without row tables (no
+        // table staging, no host-to-device copy); the same kernels as the gathered plan
+        const bool gathered = gathered_supported(type, n) && group.size() > 1;
         try {
             if (gathered) {
                 // row tables relative to the lowest input / output address of the group
