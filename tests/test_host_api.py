@@ -189,6 +189,26 @@ def test_plan_layouts():
     assert d["workspace"] > 0  # the input is not modified, so the complex axis goes to workspace
 
 
+def test_single_plane_kernel_choices():
+    """The launch shapes the measured defaults rely on: one LDS plane for contiguous C2C rows, the
+    fused real-transform rows, column tiles (with the four-step twiddle on store for stage A) and the
+    128 x 128 slice; two planes for the four-step row stage of 16 threads per transform."""
+    st = dry([4096], S.ASCEND_FFT_C2C, 16384)["stages"][0]
+    assert (st["split_lds"], st["lds"], st["block"], st["F"]) == (1, 17404, 256, 1)
+    for t in (S.ASCEND_FFT_R2C, S.ASCEND_FFT_C2R):
+        st = dry([8192], t, 4096)["stages"][0]
+        assert st["split_lds"] == 1 and st["L"] == 4096 and st["lds"] < 20 * 1024
+    rows, cols = dry([1024, 1024], S.ASCEND_FFT_C2C, 8)["stages"]
+    assert rows["split_lds"] == 1 and cols["split_lds"] == 1 and cols["col_direct"] == 1
+    assert cols["F"] == 16 and cols["lds"] <= 80 * 1024  # two 1024-thread tiles per CU
+    a, b = dry([65536], S.ASCEND_FFT_C2C, 3)["stages"]
+    assert a["split_lds"] == 1 and a["col_direct"] == 1 and a["tw4"] == 65536 and b["split_lds"] == 0
+    st = dry([128, 128, 128], S.ASCEND_FFT_C2C, 4)["stages"][0]
+    assert st["tile2d"] == [128, 128] and st["lds"] <= 80 * 1024
+    cols = dry([1000, 1000], S.ASCEND_FFT_C2C, 2)["stages"][1]
+    assert cols["split_lds"] == 1 and cols["F"] == 16
+
+
 def test_registry_is_thread_safe():
     L = S.lib()
     errors = []
