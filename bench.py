@@ -11,9 +11,10 @@ around the timed steps; peak = 8 TB/s HBM. traffic (HBM bytes per launch from ro
 FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel's reading) is read from
 profiles/traffic_c2c_4096x16384.json when present.
 
-cpu_baseline: the oracle's FP32 CPU FFT (same staged Stockham algorithm) on the host cores, median of
-runs over the full 16384 x 4096 batch; the leg also checks a sample of the GPU's output rows against
-the FP64 oracle. Secondary configurations (R2C/C2R 8192 x 4096, C2C 1536 / 2000 x 8192, C2C 1024 x 256,
+cpu_baseline: the oracle's FP32 CPU FFT (same staged Stockham algorithm) on the host cores (every core
+the process may use, at most 16), median of runs over the full 16384 x 4096 batch, and the FP64
+oracle FFT on 4096 of the rows; the leg also checks a sample of the GPU's output rows against the
+FP64 oracle. Secondary configurations (R2C/C2R 8192 x 4096, C2C 1536 / 2000 x 8192, C2C 1024 x 256,
 2D 1024^2 x 8; and 1 GiB arrays beyond the last-level cache: four-step 65536 x 2048, 2D 1024^2 x 128,
 3D 128^3 x 64) are timed the same way and reported under "secondary"; the precision, batching and
 serving forms have legs of their own.
@@ -43,8 +44,8 @@ def parse():
     ap.add_argument("--no-batching", action="store_true", help="skip the uneven-batch (batching form) leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-form leg")
     ap.add_argument("--no-precision", action="store_true", help="skip the precision-form (MFMA) leg")
-    ap.add_argument("--serving-seconds", type=float, default=2.0,
-                    help="arrival window per rate (the serving configuration specifies 60 s)")
+    ap.add_argument("--serving-seconds", type=float, default=60.0,
+                    help="arrival window per rate (the serving configuration's 60 s by default)")
     return ap.parse_args()
 
 
@@ -162,6 +163,7 @@ def bench_batching(torch, S, stream, reps=5):
     for name, ms in (("per_problem_plain", ms_a), ("grouped_direct", ms_b), ("graph_autotuned", ms_c)):
         res[name] = {"ms": round(ms, 4), "GFLOP/s": round(flops / (ms * 1e-3) / 1e9, 1),
                      "hbm_GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+    res["graph_vs_grouped_direct_speedup"] = round(ms_b / ms_c, 3)
     res["graph_vs_per_problem_speedup"] = round(ms_a / ms_c, 3)
     res["groups"] = d["groups"]
     res["chains"] = d["chains"]
@@ -174,38 +176,48 @@ def bench_batching(torch, S, stream, reps=5):
 def bench_precision(torch, S, stream, steps):
     """Precision form next to the plain form: the same C2C plans with the radix-16 stages on the
     matrix cores (FP16 hi+lo, BF16 hi+lo, BF16 hi+mid+lo), GFLOP/s (5 N log2 N) and device time per
-    exec; for the 1D headline shape also rel-L2 vs the FP64 oracle on 64 sampled rows."""
+    exec. Inputs: uniform[-1, 1) for the 1D headline shape (rel-L2 / max-abs vs the FP64 oracle on 64
+    sampled rows); the precision configuration's Gaussian N(0,1) + 3 tones of amplitude 1e3 at random
+    bins for the 2D 1024^2 x 8 and 3D 128^3 x 4 cases (rel-L2 / max-abs vs the FP64 oracle over the
+    whole array), per dtype."""
     import numpy as np
     import oracle
     modes = [("fp32", 0), ("f16x2", S.PREC_F16_SPLIT), ("bf16x2", S.PREC_BF16_SPLIT),
              ("bf16x3", S.PREC_BF16_SPLIT3)]
-    cases = [("c2c_4096x16384", [4096], 16384), ("c2c_2d_1024x1024x8", [1024, 1024], 8),
-             ("c2c_3d_128x128x128x4", [128, 128, 128], 4)]
+    cases = [("c2c_4096x16384", [4096], 16384, None), ("c2c_2d_1024x1024x8", [1024, 1024], 8, 0x5EED0010),
+             ("c2c_3d_128x128x128x4", [128, 128, 128], 4, 0x5EED0011)]
     res = {}
-    for name, dims, b in cases:
+    for name, dims, b, tone_seed in cases:
         nn = int(math.prod(dims))
-        with torch.cuda.stream(stream):
-            g = torch.Generator(device="cuda")
-            g.manual_seed(0x5EED0005)
-            xi = torch.view_as_complex(torch.rand([b] + dims + [2], generator=g, device="cuda") * 2 - 1).contiguous()
-            yo = torch.empty_like(xi)
-        torch.cuda.synchronize()
-        fl = 5.0 * nn * math.log2(nn) * b
-        rows = None
-        if len(dims) == 1:
+        if tone_seed is None:
+            with torch.cuda.stream(stream):
+                g = torch.Generator(device="cuda")
+                g.manual_seed(0x5EED0005)
+                xi = torch.view_as_complex(torch.rand([b] + dims + [2], generator=g, device="cuda") * 2 - 1).contiguous()
             rows = np.arange(0, b, b // 64)
             xs = xi.cpu().numpy()[rows]
             ref = oracle.c2c(xs)
-        entry = {}
+        else:
+            xh = oracle.tones_complex([b] + dims, seed=tone_seed)
+            rows = None
+            ref = oracle.c2c_nd(xh, len(dims))
+            xi = torch.from_numpy(xh).to("cuda")
+        with torch.cuda.stream(stream):
+            yo = torch.empty_like(xi)
+        torch.cuda.synchronize()
+        fl = 5.0 * nn * math.log2(nn) * b
+        entry = {"input": "uniform[-1,1), 64 sampled rows checked" if rows is not None
+                 else "N(0,1) + 3 tones of 1e3 at random bins, whole array checked"}
         for mname, mode in modes:
             hh = make_handle(S, torch, dims, S.ASCEND_FFT_C2C, b, stream, mode)
             st = max(10, steps // 2)
             w, kms = timed_exec(torch, hh, "c2c", xi, yo, st, 3, stream)
             e = {"GFLOP/s": round(fl * st / w / 1e9, 1), "device_ms": round(kms, 5),
                  "device_GFLOP/s": round(fl / (kms * 1e-3) / 1e9, 1)}
-            if rows is not None:
-                rel, mx = oracle.error(yo.cpu().numpy()[rows], ref)
-                e["rel_l2_sample"] = float(f"{rel:.3e}")
+            got = yo.cpu().numpy()
+            rel, mx = oracle.error(got[rows] if rows is not None else got, ref)
+            e["rel_l2"] = float(f"{rel:.3e}")
+            e["max_abs"] = float(f"{mx:.3e}")
             entry[mname] = e
             hh.destroy()
         res[name] = entry
@@ -223,10 +235,11 @@ def bench_serving(torch, S, seconds, rates=(2000.0, 10000.0, 50000.0)):
     pool = torch.rand(256 << 20, device="cuda") * 2 - 1          # 1 GiB pool
     torch.cuda.synchronize()
     res = {"config": {"c2c_frac": 0.7, "n": [256, 32768], "batch": [1, 512], "budget_GB": 64, "streams": 4,
-                      "window_s": seconds, "note": "the configuration specifies 60 s per rate"}}
+                      "window_s": seconds}}
     # Cold start: the first request of each (type, n) builds and uploads that shape's coefficient
     # tables (the server then keeps them resident). A one-second stream at a moderate rate is served
     # first and reported on its own, so the rates below measure the steady state.
+    print("[bench] serving: cold start (5000 req/s, 1 s)", file=sys.stderr, flush=True)
     r = srv.benchmark(rate=5000.0, seconds=1.0, pool_ptr=pool.data_ptr(), pool_bytes=pool.numel() * 4,
                       seed=7, drain_seconds=5.0)
     res["cold_start_5000"] = {"completed": r["completed"], "p50_ms": round(r["server"]["p50_ms"], 2),
@@ -234,6 +247,7 @@ def bench_serving(torch, S, seconds, rates=(2000.0, 10000.0, 50000.0)):
                               "note": "first 1 s stream of a fresh server: includes building every shape's tables"}
     cap = None
     for i, rate in enumerate(rates):
+        print(f"[bench] serving: {rate:.0f} req/s for {seconds:.0f} s", file=sys.stderr, flush=True)
         r = srv.benchmark(rate=rate, seconds=seconds, pool_ptr=pool.data_ptr(), pool_bytes=pool.numel() * 4,
                           seed=100 + i, drain_seconds=seconds)
         sv = r["server"]
@@ -248,8 +262,10 @@ def bench_serving(torch, S, seconds, rates=(2000.0, 10000.0, 50000.0)):
                                      "arena_peak_GB": round(sv["arena_peak"] / 1e9, 2)}
         cap = max(cap or 0.0, r["achieved_req_per_s"])
     sustain = max(1.0, 0.5 * (cap or 2.0))
-    r = srv.benchmark(rate=sustain, seconds=max(seconds, 2.0), pool_ptr=pool.data_ptr(), pool_bytes=pool.numel() * 4,
-                      seed=99, drain_seconds=max(seconds, 2.0) * 2)
+    sus_s = min(seconds, 10.0)
+    print(f"[bench] serving: sustainable {sustain:.0f} req/s for {sus_s:.0f} s", file=sys.stderr, flush=True)
+    r = srv.benchmark(rate=sustain, seconds=sus_s, pool_ptr=pool.data_ptr(), pool_bytes=pool.numel() * 4,
+                      seed=99, drain_seconds=2 * sus_s)
     sv = r["server"]
     res["sustainable"] = {"offered_req_s": round(sustain, 1), "completed": r["completed"],
                           "dropped_unadmitted": r["dropped_unadmitted"], "p50_ms": round(sv["p50_ms"], 2),
@@ -338,6 +354,7 @@ def main():
     }
 
     if not args.no_secondary:
+        print("[bench] secondary", file=sys.stderr, flush=True)
         sec = {}
         cases = [("c2c_1024x256", [1024], S.ASCEND_FFT_C2C, 256),
                  ("c2c_1536x8192", [1536], S.ASCEND_FFT_C2C, 8192),
@@ -378,21 +395,27 @@ def main():
         settle(torch, "secondary")
 
     if not args.no_precision:
+        print("[bench] precision", file=sys.stderr, flush=True)
         out["precision"] = bench_precision(torch, S, stream, args.steps)
         settle(torch, "precision")
 
     if not args.no_batching:
+        print("[bench] batching", file=sys.stderr, flush=True)
         out["batching"] = bench_batching(torch, S, stream)
         settle(torch, "batching")
 
     if not args.no_serving:
+        print("[bench] serving", file=sys.stderr, flush=True)
         out["serving"] = bench_serving(torch, S, args.serving_seconds)
 
     if not args.no_cpu:
         import numpy as np
         import oracle
-        cores = len(os.sched_getaffinity(0))
-        threads = min(cores, 16)
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        visible = len(os.sched_getaffinity(0))
+        # worker threads: every core this process may use, capped at 16 (the host share of one GPU
+        # on an MI355X node; OMP_NUM_THREADS on the box says the same)
+        threads = min(visible, 16)
         xs = x.cpu().numpy()
         ys = y.cpu().numpy()
         oracle.c2c_f32(xs[:256], threads=threads)  # warm-up
@@ -402,13 +425,26 @@ def main():
             oracle.c2c_f32(xs, threads=threads)
             times.append(time.perf_counter() - t0)
         tmed = sorted(times)[len(times) // 2]
+        # FP64 oracle FFT (OpenMP over the batch) on a bounded sample: 4096 of the 16384 rows
+        sample = xs[:4096]
+        oracle.c2c(sample[:64])
+        t64 = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            oracle.c2c(sample)
+            t64.append(time.perf_counter() - t0)
+        t64m = sorted(t64)[1]
+        fl64 = 5.0 * n * math.log2(n) * sample.shape[0]
         rows = np.arange(0, batch, batch // 64)
         rel, mx = oracle.error(ys[rows], oracle.c2c(xs[rows]))
         out["cpu_baseline"] = {"value": round(flops / tmed / 1e9, 2), "unit": "GFLOP/s", "cores": threads,
+                               "cores_visible": visible, "omp_threads_fp64": oracle.max_threads(),
                                "kind": "port",
-                               "sample": "full C2C N=4096 x batch 16384, FP32 staged Stockham (oracle/fft_oracle.c), "
-                                         "OpenMP over the batch, median of 5 runs",
+                               "sample": "FP32: full C2C N=4096 x batch 16384, staged Stockham (oracle/fft_oracle.c), "
+                                         "OpenMP over the batch on `cores` threads, median of 5 runs; FP64: the "
+                                         "oracle's FFT on the first 4096 rows, median of 3",
                                "seconds_per_batch": round(tmed, 3),
+                               "fp64_value": round(fl64 / t64m / 1e9, 2), "fp64_seconds_4096_rows": round(t64m, 3),
                                "gpu_rel_l2_vs_fp64_sample": rel, "gpu_max_abs_vs_fp64_sample": mx,
                                "gpu_check_rows": int(rows.size)}
     h.destroy()

@@ -1,6 +1,11 @@
 /*
  * asdFft* public API of the MI355X FFT library (C++ view).
  *
+ * The declarations reproduce the interface of AscendSiPBoost's include/fft_api.h
+ * (Copyright (c) 2025 Huawei Technologies Co., Ltd.; CANN Open Software License Agreement
+ * Version 2.0) so that this library is a drop-in for it; the comments and everything behind the
+ * declarations are this library's own.
+ *
  * Same entry points, argument meaning, enum values and status codes as the reference's
  * include/fft_api.h:15-101 (declared there inside extern "C" { namespace AsdSip { ... } } with
  * C++ references and one default argument). The symbols are unmangled C names; a reference is
@@ -61,7 +66,8 @@ AsdSip::AspbStatus asdFftMakePlan1D(asdFftHandle handle, int64_t fftSize, asdFft
                                     asdFftDirection direction, int64_t batchSize,
                                     asdFft1dDimType dimType = asdFft1dDimType::ASCEND_FFT_HORIZONTAL);
 
-// ISTFT plan (fft_api.h:59-62). Returns ACL_ERROR_API_NOT_SUPPORT in this build.
+// ISTFT plan (fft_api.h:59-62): the reference's parameter checks and status codes; the inverse DFTs
+// and the windowed overlap-add run on the GPU.
 AspbStatus asdFftIstftMakePlan(asdFftHandle handle, const aclTensor *input, const int64_t nFft,
                                const int64_t hopLengthOpt, const int64_t winLengthOpt, const bool center = true,
                                const bool normalized = false, const bool onesidedOpt = false,
@@ -80,7 +86,7 @@ AsdSip::AspbStatus asdFftExecC2R(asdFftHandle handle, const aclTensor *input, co
 // R2C execution (fft_api.h:75-76).
 AsdSip::AspbStatus asdFftExecR2C(asdFftHandle handle, const aclTensor *input, const aclTensor *output);
 
-// ISTFT execution (fft_api.h:78-79). Returns ACL_ERROR_API_NOT_SUPPORT in this build.
+// ISTFT execution (fft_api.h:78-79) of a plan made by asdFftIstftMakePlan.
 AspbStatus asdFftExecIstft(asdFftHandle handle, const aclTensor *input,
                            const aclTensor *windowOpt, const aclTensor *output);
 

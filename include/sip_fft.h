@@ -63,8 +63,7 @@ AspbStatus asdFftDestroy(asdFftHandle handle);
 /* ------------------------------------------------------------------ extensions (no counterpart
  * in the reference; used by the tests, the benchmark and the batching / serving layers) */
 
-/* Writes a JSON description of a handle's plan (stages, kernels, radices, workspace, and the core
- * the reference would have selected) into buf. Returns 0, 100000 on a bad handle, or 100000 if
+/* Writes a JSON description of a handle's plan (stages, kernels, radices, workspace) into buf. Returns 0, 100000 on a bad handle, or 100000 if
  * cap is too small (then *needed holds the size including the terminating NUL). */
 int32_t sipFftPlanDescribe(void *handle, char *buf, size_t cap, size_t *needed);
 
@@ -72,6 +71,19 @@ int32_t sipFftPlanDescribe(void *handle, char *buf, size_t cap, size_t *needed);
  * needed): ndim 1..3, dims[ndim], type/direction/dimType as for asdFftMakePlan*. */
 int32_t sipFftPlanDescribeDry(int32_t ndim, const int64_t *dims, int32_t fftType, int32_t direction,
                               int64_t batchSize, int32_t dimType, char *buf, size_t cap, size_t *needed);
+
+/* Host-generated content of one coefficient table of a plan laid out on the host (no GPU needed),
+ * as interleaved (re, im) floats: stage = index into the plan's stages (as described by
+ * sipFftPlanDescribeDry); which = 0 pass twiddles, 1 real-transform roots, 2 / 3 four-step
+ * twiddles (low / high level), 4 auxiliary table (Bluestein chirp or kernel spectrum), 5 column pass
+ * twiddles of a fused 2D tile. *count receives the number of complex entries; cap is the capacity
+ * of out in floats (at least 2 * count). Returns 100000 for a table the stage does not have. */
+int32_t sipFftPlanTableDry(int32_t ndim, const int64_t *dims, int32_t fftType, int32_t direction, int64_t batchSize,
+                           int32_t dimType, int32_t stage, int32_t which, float *out, size_t cap, size_t *count);
+
+/* Device-resident coefficient cache: map entries (live or released), live tables and their bytes.
+ * A table is shared by every live plan that uses it and freed with the last of them. */
+int32_t sipFftCoeffCacheStats(size_t *entries, size_t *live_tables, size_t *live_bytes);
 
 /* Library version string. */
 const char *sipFftVersion(void);

@@ -1,6 +1,8 @@
 /*
  * Status codes returned by every asdFft* entry point.
- * Same names and values as the reference (core/utils/include/utils/aspb_status.h:16-27).
+ * Same names and values as AscendSiPBoost's core/utils/include/utils/aspb_status.h:16-27
+ * (Copyright (c) 2025 Huawei Technologies Co., Ltd.; CANN Open Software License Agreement
+ * Version 2.0), reproduced so that callers see the reference's codes.
  */
 #ifndef ASPB_STATUS_H
 #define ASPB_STATUS_H

@@ -11,6 +11,9 @@ forward uses e^{-2 pi i kn/N}, inverse e^{+2 pi i kn/N} with no 1/N scaling; R2C
 0..N/2; C2R transforms the Hermitian extension of N/2+1 bins and keeps the real part; 2D/3D
 transforms are separable, R2C doing the real axis (last) first and C2R last.
 
+``reference_steps`` / ``reference_istft_steps`` report which core and step list the reference
+would build for an asdFft* call (``oracle/ref_rules.cpp``, built into ``oracle/libref_rules.so``).
+
 Every function here is pinned by tests/test_oracle.py against closed forms (impulse, constant,
 pure tones, the ramp the reference's examples feed in), numpy.fft, and the brute-force DFT; the
 ISTFT restatement against torch.istft (an independent implementation of the same definition), STFT
@@ -26,16 +29,57 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_RULES_PATH = os.path.join(_HERE, "libref_rules.so")
 _lib = None
+_rules = None
 
 FORWARD = -1
 INVERSE = +1
 
 
 def build() -> str:
-    """Compile liboracle.so from fft_oracle.c (gcc + OpenMP)."""
+    """Compile liboracle.so from fft_oracle.c (gcc + OpenMP) and libref_rules.so from ref_rules.cpp."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
     return _LIB_PATH
+
+
+def _rules_lib():
+    global _rules
+    if _rules is None:
+        if not os.path.exists(_RULES_PATH):
+            build()
+        L = ctypes.CDLL(_RULES_PATH)
+        i64, sz = ctypes.c_int64, ctypes.c_size_t
+        L.ref_steps_json.argtypes = [ctypes.c_int, ctypes.POINTER(i64), ctypes.c_int, ctypes.c_int, i64, ctypes.c_int,
+                                     ctypes.c_char_p, sz, ctypes.POINTER(sz)]
+        L.ref_istft_steps_json.argtypes = [i64, i64, i64, i64, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
+        _rules = L
+    return _rules
+
+
+def _json_call(call):
+    import json
+    need = ctypes.c_size_t()
+    call(None, 0, ctypes.byref(need))
+    buf = ctypes.create_string_buffer(need.value)
+    if call(buf, need.value, ctypes.byref(need)) != 0:
+        raise RuntimeError("reference rules: buffer too small")
+    return json.loads(buf.value.decode())
+
+
+def reference_steps(dims, fft_type, batch, dim_type=0x10, direction=0x10):
+    """The step list the reference builds for asdFftMakePlan{1,2,3}D with these arguments
+    (core names and transposes, e.g. ["FftB(N=4096,batch=16384,radix=[64,64])"])."""
+    dims = [int(d) for d in dims]
+    arr = (ctypes.c_int64 * len(dims))(*dims)
+    return _json_call(lambda buf, cap, need: _rules_lib().ref_steps_json(len(dims), arr, fft_type, direction, batch,
+                                                                          dim_type, buf, cap, need))
+
+
+def reference_istft_steps(channel, n_fft, frames, hop):
+    """The reference's ISTFT step list (core/fft/istft.cpp:91-104)."""
+    return _json_call(lambda buf, cap, need: _rules_lib().ref_istft_steps_json(channel, n_fft, frames, hop, buf, cap,
+                                                                                need))
 
 
 def lib():

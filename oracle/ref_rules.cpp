@@ -1,6 +1,7 @@
-// Restatement of the reference's plan rules, used only to describe which core and step list the
-// reference would build for the same call (reported by sipFftPlanDescribe next to this library's
-// own stages). Nothing here drives this library's kernels.
+// Restatement of the reference's plan rules: which core and step list the reference would build
+// for a given asdFft* call. Test infrastructure only (built into oracle/libref_rules.so by
+// oracle/Makefile and called from the CPU tests); nothing here is part of libsip_fft.so or drives
+// its kernels.
 //
 // Sources: ChooseRadix / Support / deDuplicates / orderedFactorize (core/fft/fft_api.cpp:66-110,
 // core/fft/utils/include/utils/fft_common_func.h:39-62), getC2CCore / getR2CCore / getC2RCore /
@@ -9,15 +10,25 @@
 // Fft2dSupportFusing / addFFTC2CFirstTwoDimsStep (fft_api.cpp:112-167, 527-697), FFTCoreB::InitRadix
 // (core/fft/fftcore/fft_core_b.cpp:126-168), FindTwoRadix / InitMixRadixShort
 // (core/fft/fftcore/fft_core_common_func.cpp:113-144, 767-826; RADIX_ADDR_START = 128 from
-// fft_core_common_func.h:43), and the R2C/C2R half length (fft_r2c_core.h:20-24).
+// fft_core_common_func.h:43), the R2C/C2R half length (fft_r2c_core.h:20-24) and the ISTFT step
+// list (core/fft/istft.cpp:91-104).
+#include <stddef.h>
+#include <stdint.h>
+
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <sstream>
+#include <string>
+#include <vector>
 
-#include "plan.h"
-
-namespace sip {
+namespace refrules {
 namespace {
+
+// enum values of the reference's include/fft_api.h:26-44
+constexpr int kTypeC2C = 0x10, kTypeC2R = 0x11, kTypeR2C = 0x12, kTypeC2CSep = 0x30;
+constexpr int kDirInverse = 0x11;
+constexpr int kDimVertical = 0x11;
 
 constexpr int kRadix2 = 2, kRadixMix = -2, kRadixAny = -1;
 
@@ -348,4 +359,30 @@ std::vector<std::string> reference_steps(int type, int direction, const std::vec
     return s;
 }
 
-}  // namespace sip
+}  // namespace refrules
+
+// C entry points for the tests: the step list as a JSON array of strings. Returns 0, or 1 when
+// cap is too small (needed holds the size either way).
+static int write_json(const std::vector<std::string> &steps, char *buf, size_t cap, size_t *needed)
+{
+    std::string j = "[";
+    for (size_t i = 0; i < steps.size(); i++) j += (i ? ",\"" : "\"") + steps[i] + "\"";
+    j += "]";
+    if (needed) *needed = j.size() + 1;
+    if (!buf || cap < j.size() + 1) return 1;
+    std::memcpy(buf, j.c_str(), j.size() + 1);
+    return 0;
+}
+
+extern "C" int ref_steps_json(int ndim, const int64_t *dims, int type, int direction, int64_t batch, int dim_type,
+                              char *buf, size_t cap, size_t *needed)
+{
+    std::vector<int64_t> sizes(dims, dims + ndim);
+    return write_json(refrules::reference_steps(type, direction, sizes, batch, dim_type), buf, cap, needed);
+}
+
+extern "C" int ref_istft_steps_json(int64_t channel, int64_t nfft, int64_t frames, int64_t hop, char *buf, size_t cap,
+                                    size_t *needed)
+{
+    return write_json(refrules::reference_istft_steps(channel, nfft, frames, hop), buf, cap, needed);
+}

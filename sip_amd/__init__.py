@@ -53,7 +53,8 @@ EXPORTED_SYMBOLS = [
     "sipFftBatchCreate", "sipFftBatchSetStream", "sipFftBatchMakePlan", "sipFftBatchGetWorkspaceSize",
     "sipFftBatchSetWorkspace", "sipFftBatchExec", "sipFftBatchDescribe", "sipFftBatchDestroy",
     "sipFftServeCreate", "sipFftServeSubmit", "sipFftServeWait", "sipFftServeDrain", "sipFftServeStats",
-    "sipFftServeBenchmark", "sipFftServeDestroy", "sipFftSetPrecision",
+    "sipFftServeBenchmark", "sipFftServeDestroy", "sipFftSetPrecision", "sipFftPlanTableDry",
+    "sipFftCoeffCacheStats",
 ]
 
 PREC_FP32, PREC_F16_SPLIT, PREC_BF16_SPLIT, PREC_BF16_SPLIT3 = 0, 1, 2, 3
@@ -117,6 +118,8 @@ def lib():
                                  i64, i64, i64, ctypes.c_double, ctypes.c_char_p, sz, P(sz)],
         "sipFftServeDestroy": [vp],
         "sipFftSetPrecision": [vp, i32],
+        "sipFftPlanTableDry": [i32, P(i64), i32, i32, i64, i32, i32, i32, P(ctypes.c_float), sz, P(sz)],
+        "sipFftCoeffCacheStats": [P(sz), P(sz), P(sz)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -348,6 +351,31 @@ def describe_dry(dims, fft_type, direction, batch, dim_type=ASCEND_FFT_HORIZONTA
     arr = (ctypes.c_int64 * len(dims))(*dims)
     return json.loads(_describe(lambda buf, cap, need: lib().sipFftPlanDescribeDry(
         len(dims), arr, fft_type, direction, batch, dim_type, buf, cap, need)))
+
+
+def plan_table_dry(dims, fft_type, direction, batch, stage, which, dim_type=ASCEND_FFT_HORIZONTAL):
+    """Host-generated coefficient table `which` (0 pass twiddles, 1 real roots, 2 / 3 four-step low /
+    high, 4 auxiliary, 5 fused-2D column twiddles) of stage `stage` of a host-only plan, as complex64."""
+    import numpy as np
+    dims = [int(d) for d in dims]
+    arr = (ctypes.c_int64 * len(dims))(*dims)
+    cnt = ctypes.c_size_t()
+    st = lib().sipFftPlanTableDry(len(dims), arr, fft_type, direction, batch, dim_type, stage, which, None, 0,
+                                  ctypes.byref(cnt))
+    if cnt.value == 0:
+        _check(st, "sipFftPlanTableDry")
+    out = np.empty(2 * cnt.value, dtype=np.float32)
+    _check(lib().sipFftPlanTableDry(len(dims), arr, fft_type, direction, batch, dim_type, stage, which,
+                                    out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), out.size, ctypes.byref(cnt)),
+           "sipFftPlanTableDry")
+    return out.view(np.complex64)
+
+
+def coeff_cache_stats():
+    """(map entries, live tables, live bytes) of the device-resident coefficient cache."""
+    e, n, b = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
+    _check(lib().sipFftCoeffCacheStats(ctypes.byref(e), ctypes.byref(n), ctypes.byref(b)), "sipFftCoeffCacheStats")
+    return e.value, n.value, b.value
 
 
 class BatchHandle:

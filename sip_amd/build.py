@@ -35,12 +35,28 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-D__HIP_PLATFORM_AMD__", "-I" + os.
               "-Wall", "-Wno-unused-parameter"] + INCLUDES
 
 
-def _headers():
-    hs = []
-    for pat in ("include/**/*.h", "sip_amd/csrc/**/*.h"):
-        hs += glob.glob(os.path.join(ROOT, pat), recursive=True)
-    hs += glob.glob(os.path.join(GEN, "*.inc"))
-    return sorted(hs)
+_INC_DIRS = [os.path.join(ROOT, "include"), os.path.join(CSRC, "kernels"), GEN, os.path.join(CSRC, "host")]
+
+
+def _headers(src, seen=None):
+    """Project headers reached from src through #include "..." lines (recursively), so that an
+    object is rebuilt only when a header it really includes changes."""
+    seen = set() if seen is None else seen
+    here = os.path.dirname(src)
+    with open(src, errors="replace") as fh:
+        for line in fh:
+            line = line.strip()
+            if not line.startswith("#include") or '"' not in line:
+                continue
+            name = line.split('"')[1]
+            for d in [here] + _INC_DIRS:
+                cand = os.path.normpath(os.path.join(d, name))
+                if os.path.exists(cand):
+                    if cand not in seen:
+                        seen.add(cand)
+                        _headers(cand, seen)
+                    break
+    return sorted(seen)
 
 
 def _digest(src, flags, headers):
@@ -77,18 +93,17 @@ def build(force: bool = True, jobs: int | None = None, verbose: bool = False, ex
     dflags = DEVICE_FLAGS + list(extra_device_flags)
     subprocess.run([sys.executable, os.path.join(CSRC, "kernels", "gen_instances.py"), GEN, str(NPARTS)],
                    check=True, capture_output=not verbose)
-    headers = _headers()
     work = []
     dev_srcs = sorted(glob.glob(os.path.join(CSRC, "kernels", "*.hip"))) + sorted(glob.glob(os.path.join(GEN, "*.hip")))
     host_srcs = sorted(glob.glob(os.path.join(CSRC, "host", "*.cpp")))
     for src in dev_srcs:
         obj = os.path.join(obj_root, os.path.basename(src) + ".o")
         cmd = [HIPCC] + dflags + ["-c", src, "-o", obj]
-        work.append((src, obj, cmd, _digest(src, dflags, headers), force))
+        work.append((src, obj, cmd, _digest(src, dflags, _headers(src)), force))
     for src in host_srcs:
         obj = os.path.join(obj_root, os.path.basename(src) + ".o")
         cmd = [CLANGXX] + HOST_FLAGS + ["-c", src, "-o", obj]
-        work.append((src, obj, cmd, _digest(src, HOST_FLAGS, headers), force))
+        work.append((src, obj, cmd, _digest(src, HOST_FLAGS, _headers(src)), force))
     jobs = jobs or min(8, os.cpu_count() or 4)
     errors = []
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:

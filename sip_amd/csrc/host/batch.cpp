@@ -6,9 +6,9 @@
 // member through a device row-offset table, so no data is copied or packed. The groups are spread
 // over a few parallel chains (longest-processing-time first) and captured once into a hipGraph;
 // each exec is one graph launch. An optional autotuner times the runtime tile-shape choices of every
-// group (transforms per workgroup, LDS layout, tile order, four-step split) on the real buffers and
-// keeps the fastest, caching decisions in a JSON file keyed by length, row count, direction and
-// device. This layer has no counterpart in the reference; it is an extension API (sipFftBatch*).
+// group (transforms per workgroup, LDS layout, tile order, four-step split) on scratch rows of the
+// group's shape and keeps the fastest, caching decisions in a JSON file keyed by length, row count,
+// direction and device. This layer has no counterpart in the reference; it is an extension API (sipFftBatch*).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -171,6 +171,24 @@ std::unique_ptr<Plan> build_group_plan(const BatchPlan &bp, const Group &g, cons
     return p;
 }
 
+// The plan the tuner times for a group: the same length, row count and tile choices, on compact
+// scratch rows (row r at element r * n of scratch_in / scratch_out). The caller's buffers are never
+// touched at plan time (an in-place problem would otherwise be transformed by every timing run).
+std::unique_ptr<Plan> build_tuning_plan(const BatchPlan &bp, const Group &g, const Tune &t, void *scratch_in,
+                                        void *scratch_out)
+{
+    auto p = std::make_unique<Plan>();
+    p->direction = bp.direction;
+    p->tune = t;
+    std::vector<int64_t> rows((size_t)g.rows);
+    for (int64_t r = 0; r < g.rows; r++) rows[(size_t)r] = r * g.n;
+    plan_build_gathered_c2c(*p, g.n, rows, rows, scratch_in, scratch_out);
+    if (!plan_launchable(*p)) throw SipError(AsdSip::ACL_ERROR_API_NOT_SUPPORT, "candidate exceeds launch limits");
+    plan_materialize(*p);
+    p->initialized = true;
+    return p;
+}
+
 // Candidate tile-shape choices for one group.
 std::vector<Tune> candidates(int64_t n)
 {
@@ -236,7 +254,12 @@ float time_plan(Plan &p, hipStream_t s, int reps)
     return ms * 1000.f / reps;
 }
 
-void tune_group(BatchPlan &bp, Group &g, void *scratch_ws, size_t scratch_bytes)
+struct TuneScratch {
+    void *in = nullptr, *out = nullptr, *ws = nullptr;
+    size_t ws_bytes = 0;
+};
+
+void tune_group(BatchPlan &bp, Group &g, const TuneScratch &sc)
 {
     const std::string key = tune_key(g.n, g.rows, bp.direction);
     {
@@ -253,12 +276,12 @@ void tune_group(BatchPlan &bp, Group &g, void *scratch_ws, size_t scratch_bytes)
     for (const Tune &t : candidates(g.n)) {
         std::unique_ptr<Plan> p;
         try {
-            p = build_group_plan(bp, g, t);
+            p = build_tuning_plan(bp, g, t, sc.in, sc.out);
         } catch (const SipError &) {
-            continue;
+            continue;  // not plannable or beyond the device's launch limits: never launched
         }
-        if (p->ws_bytes > scratch_bytes) continue;
-        p->workspace = scratch_ws;
+        if (p->ws_bytes > sc.ws_bytes) continue;
+        p->workspace = sc.ws;
         const float us = time_plan(*p, bp.stream, 5);  // null stream is fine for timing
         if (us < best_us) {
             best_us = us;
@@ -412,22 +435,46 @@ int32_t sipFftBatchMakePlan(void *handle, int32_t count, const int64_t *lengths,
             load[c] += g.cost + 2.0e5;  // + a fixed per-launch cost
         }
         // plans (tuned or default); tuning needs a scratch workspace sized for the largest candidate
+        // Tuning runs on scratch rows (input, output and workspace of the largest group), zeroed
+        // once; the caller's buffers are not read or written before the first exec.
         const bool autotune = (flags & 2) != 0;
         void *scratch = nullptr;
-        size_t scratch_bytes = 0;
+        sip::TuneScratch sc;
         if (autotune) {
-            for (sip::Group &g : b->groups) scratch_bytes = std::max(scratch_bytes, (size_t)(16.0 * g.n * g.rows) + 4096);
-            SIP_HIP_CHECK(hipMalloc(&scratch, scratch_bytes));
+            size_t rows_bytes = 0, ws_bytes = 0;
+            for (sip::Group &g : b->groups) {
+                rows_bytes = std::max(rows_bytes, (size_t)g.n * (size_t)g.rows * 8);
+                // four-step and Bluestein intermediates of the default layout (host-only plan)
+                sip::Plan dry;
+                dry.direction = b->direction;
+                std::vector<int64_t> rr((size_t)g.rows, 0);
+                sip::plan_build_gathered_c2c(dry, g.n, rr, rr, nullptr, nullptr);
+                ws_bytes = std::max(ws_bytes, dry.ws_bytes);
+            }
+            rows_bytes = (rows_bytes + 255) / 256 * 256;
+            sc.ws_bytes = std::max(ws_bytes, rows_bytes) + 4096;
+            SIP_HIP_CHECK(hipMalloc(&scratch, 2 * rows_bytes + sc.ws_bytes));
+            sc.in = scratch;
+            sc.out = static_cast<char *>(scratch) + rows_bytes;
+            sc.ws = static_cast<char *>(scratch) + 2 * rows_bytes;
+            const hipError_t ze = hipMemset(scratch, 0, 2 * rows_bytes + sc.ws_bytes);
+            if (ze != hipSuccess) {
+                (void)hipFree(scratch);
+                SIP_HIP_CHECK(ze);
+            }
         }
         try {
             for (sip::Group &g : b->groups) {
                 if (autotune)
-                    sip::tune_group(*b, g, scratch, scratch_bytes);
+                    sip::tune_group(*b, g, sc);
                 else
                     g.plan = sip::build_group_plan(*b, g, sip::Tune{});
             }
         } catch (...) {
-            if (scratch) (void)hipFree(scratch);
+            if (scratch) {
+                (void)hipDeviceSynchronize();
+                (void)hipFree(scratch);
+            }
             throw;
         }
         if (scratch) SIP_HIP_CHECK(hipFree(scratch));

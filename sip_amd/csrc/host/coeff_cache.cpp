@@ -37,6 +37,14 @@ TablePtr coeff_get(const TableRef &ref)
     if (it != g_cache.end()) {
         if (TablePtr live = it->second.lock()) return live;
     }
+    // entries whose last plan is gone are dropped here (tables are created at plan time only), so
+    // a long-lived process that sees many shapes keeps a map of live tables only
+    for (auto e = g_cache.begin(); e != g_cache.end();) {
+        if (e->second.expired())
+            e = g_cache.erase(e);
+        else
+            ++e;
+    }
     std::vector<float2> host = ref.gen();
     auto t = std::make_shared<DeviceTable>();
     t->bytes = host.size() * sizeof(float2);
@@ -57,4 +65,24 @@ size_t coeff_cache_live_tables()
     return n;
 }
 
+void coeff_cache_stats(size_t *entries, size_t *live, size_t *live_bytes)
+{
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    size_t n = 0, b = 0;
+    for (auto &kv : g_cache)
+        if (TablePtr t = kv.second.lock()) {
+            n++;
+            b += t->bytes;
+        }
+    if (entries) *entries = g_cache.size();
+    if (live) *live = n;
+    if (live_bytes) *live_bytes = b;
+}
+
 }  // namespace sip
+
+extern "C" int32_t sipFftCoeffCacheStats(size_t *entries, size_t *live_tables, size_t *live_bytes)
+{
+    sip::coeff_cache_stats(entries, live_tables, live_bytes);
+    return AsdSip::ACL_SUCCESS;
+}

@@ -38,14 +38,15 @@ struct SipError : std::runtime_error {
     SipError(AspbStatus s, const std::string &m) : std::runtime_error(m), status(s) {}
 };
 
-// A failing HIP call becomes a SipError (mapped to a status at the C-ABI). The runtime's sticky
-// last-error is read off first, so a failure reported through a status does not resurface in the
-// caller's next, unrelated HIP call.
+// A failing HIP call becomes a SipError (mapped to a status at the C-ABI). When the runtime's
+// last-error slot holds exactly the error this call just returned, it is read off, so a failure
+// already reported through a status does not resurface in the caller's next HIP call; any other
+// pending error is left for its owner.
 #define SIP_HIP_CHECK(EXPR)                                                                        \
     do {                                                                                           \
         hipError_t sip_e_ = (EXPR);                                                                \
         if (sip_e_ != hipSuccess) {                                                                \
-            (void)hipGetLastError();                                                               \
+            if (hipPeekAtLastError() == sip_e_) (void)hipGetLastError();                           \
             throw ::sip::SipError(::AsdSip::ACL_ERROR_INTERNAL_ERROR,                              \
                                   std::string(#EXPR) + " failed: " + hipGetErrorString(sip_e_)); \
         }                                                                                          \

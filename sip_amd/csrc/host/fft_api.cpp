@@ -60,8 +60,6 @@ int32_t finish_plan(Plan &p)
     p.initialized = false;
     p.failed = false;
     p.stages.clear();
-    p.ref_steps = p.istft.active ? sip::reference_istft_steps(p.istft.channel, p.istft.nfft, p.istft.frames, p.istft.hop)
-                                 : sip::reference_steps(p.fftType, p.direction, p.sizes, p.batch, p.dimType);
     try {
         sip::plan_build(p);
         sip::plan_materialize(p);
@@ -359,12 +357,14 @@ int32_t asdFftIstftMakePlan(void *handle, const aclTensor *input, int64_t nFft, 
         p->istft.hop = hopLengthOpt;
         p->istft.win_len = winLengthOpt;
         p->istft.out_len = nFft + hopLengthOpt * (frames - 1) - 2 * (nFft / 2);
-        const int32_t st = finish_plan(*p);
-        if (st != E::ACL_SUCCESS) {
+        try {
+            return finish_plan(*p);
+        } catch (const SipError &e) {
+            // the reference's MakePlan1DFft failure: the handle keeps no ISTFT description
+            SIP_LOG(ERROR, "istft plan: " << e.what());
             p->istft = Plan::Istft{};
-            return E::ACL_ERROR_INTERNAL_ERROR;  // the reference's MakePlan1DFft failure
+            return E::ACL_ERROR_INTERNAL_ERROR;
         }
-        return st;
     });
 }
 
@@ -478,13 +478,49 @@ int32_t sipFftPlanDescribeDry(int32_t ndim, const int64_t *dims, int32_t fftType
             if (dims[i] <= 0 || dims[i] > sip::kMaxFftSize) return E::ACL_ERROR_INVALID_PARAM;
             p.sizes.push_back(dims[i]);
         }
-        p.ref_steps = sip::reference_steps(p.fftType, p.direction, p.sizes, p.batch, p.dimType);
         sip::plan_build(p);
         return write_str(sip::plan_describe(p), buf, cap, needed);
     });
 }
 
-const char *sipFftVersion(void) { return "sip_fft 0.1 (gfx950)"; }
+int32_t sipFftPlanTableDry(int32_t ndim, const int64_t *dims, int32_t fftType, int32_t direction, int64_t batchSize,
+                           int32_t dimType, int32_t stage, int32_t which, float *out, size_t cap, size_t *count)
+{
+    return guarded([&]() -> int32_t {
+        if (ndim < 1 || ndim > 3 || dims == nullptr || batchSize <= 0) return E::ACL_ERROR_INVALID_PARAM;
+        if (!valid_type(fftType) || !valid_dir(direction)) return E::ACL_ERROR_INVALID_PARAM;
+        Plan p;
+        p.fftType = fftType;
+        p.direction = direction;
+        p.batch = batchSize;
+        p.dimType = (ndim == 1 && dimType == sip::kDimVertical) ? sip::kDimVertical : sip::kDimHorizontal;
+        for (int i = 0; i < ndim; i++) {
+            if (dims[i] <= 0 || dims[i] > sip::kMaxFftSize) return E::ACL_ERROR_INVALID_PARAM;
+            p.sizes.push_back(dims[i]);
+        }
+        sip::plan_build(p);
+        if (stage < 0 || stage >= (int32_t)p.stages.size()) return E::ACL_ERROR_INVALID_PARAM;
+        const sip::Stage &st = p.stages[(size_t)stage];
+        const sip::TableRef *t = nullptr;
+        switch (which) {
+            case 0: t = &st.tw; break;
+            case 1: t = &st.rtw; break;
+            case 2: t = &st.tw4lo; break;
+            case 3: t = &st.tw4hi; break;
+            case 4: t = &st.tab0; break;
+            case 5: t = &st.tw2; break;
+            default: return E::ACL_ERROR_INVALID_PARAM;
+        }
+        if (!t->used()) return E::ACL_ERROR_INVALID_PARAM;
+        const std::vector<float2> v = t->gen();
+        if (count) *count = v.size();
+        if (out == nullptr || cap < 2 * v.size()) return E::ACL_ERROR_INVALID_PARAM;
+        std::memcpy(out, v.data(), v.size() * sizeof(float2));
+        return E::ACL_SUCCESS;
+    });
+}
+
+const char *sipFftVersion(void) { return "sip_fft 0.2 (gfx950)"; }
 
 int32_t sipFftSetPrecision(void *handle, int32_t mode)
 {

@@ -45,8 +45,8 @@ static float2 root_f(int64_t m, int64_t n, int sign)
         s = sinl(a);
     }
     float2 r;
-    r.x = (float)(double)c;
-    r.y = (float)(double)(sign * s);
+    r.x = (float)c;  // one rounding, long double -> FP32
+    r.y = (float)(sign * s);
     return r;
 }
 
@@ -302,10 +302,11 @@ TileDesc to_tile(const Lay &l, bool *ok, int *tab_id)
     return t;
 }
 
-AuxRows to_rows(const Lay &l, bool *ok)
+AuxRows to_rows(const Lay &l, bool *ok, int *tab_id)
 {
     std::vector<Dim> d = normalize(l.dims);
     AuxRows r{};
+    *tab_id = -1;
     if (d.size() > 2) {
         *ok = false;
         return r;
@@ -316,6 +317,15 @@ AuxRows to_rows(const Lay &l, bool *ok)
     r.s1 = d[1].s;
     r.es = l.es;
     r.div_n0 = make_fastdiv((uint32_t)r.n0);
+    r.tab = nullptr;
+    r.tab_dim = -1;
+    for (int i = 0; i < 2; i++) {
+        if (d[(size_t)i].tab >= 0) {
+            if (*tab_id >= 0) *ok = false;  // one gathered dimension per side
+            *tab_id = d[(size_t)i].tab;
+            r.tab_dim = i;
+        }
+    }
     return r;
 }
 
@@ -328,6 +338,7 @@ bool env_flag(const char *name, bool dflt)
 
 constexpr int64_t kColumnLeafMax = 1024;
 constexpr int64_t kRowLeafMax = 16384;
+constexpr int64_t kFillWorkgroups = 512;  // two workgroups per CU of MI355X (256 CUs)
 
 struct Builder {
     Plan &p;
@@ -428,7 +439,14 @@ struct Builder {
             if (f <= 0) {
                 if (const char *e = std::getenv("SIP_FFT_ROW_F")) f = std::atoi(e);
             }
-            if (f >= 1 && f * T <= 1024 && (size_t)f * per <= 160 * 1024) F = f;
+            if (f >= 1 && f * T <= 1024 && (size_t)f * per <= 160 * 1024) {
+                F = f;
+            } else {
+                // Small batches: fewer transforms per workgroup until the grid has two workgroups
+                // per CU (256 CUs on MI355X), down to one wave per workgroup (1024 x 256: 256
+                // workgroups of one transform instead of 64 of four).
+                while (F > 1 && ((F + 1) / 2) * T >= 64 && (nfft + F - 1) / F < kFillWorkgroups) F = (F + 1) / 2;
+            }
         }
         if (mf) {
             // every wave owns 4 blocks of 16 butterflies: F * T must be a multiple of 64
@@ -535,8 +553,8 @@ struct Builder {
         AuxArgs &x = st.xa;
         std::memset(&x, 0, sizeof(x));
         x.mode = mode;
-        x.rin = to_rows(in, &ok);
-        x.rout = to_rows(out, &ok);
+        x.rin = to_rows(in, &ok, &st.in_tab);
+        x.rout = to_rows(out, &ok, &st.out_tab);
         if (!ok) throw SipError(AsdSip::ACL_ERROR_API_NOT_SUPPORT, "aux layout has more than 2 batch dimensions");
         x.rows = count(normalize(in.dims));
         x.n = n;
@@ -971,6 +989,23 @@ static const void *stage_kernel(const Stage &s)
     return s.sign < 0 ? s.inst->fwd : s.inst->inv;
 }
 
+bool plan_launchable(const Plan &p)
+{
+    // gfx950: at most 1024 threads and 160 KiB of LDS (static + dynamic) per workgroup
+    constexpr size_t kLdsPerWorkgroup = 160 * 1024;
+    for (const Stage &s : p.stages) {
+        if (s.kind != StageKind::Stockham) continue;
+        hipFuncAttributes attr;
+        if (hipFuncGetAttributes(&attr, stage_kernel(s)) != hipSuccess) {
+            (void)hipGetLastError();  // the query's own failure, raised just above
+            return false;
+        }
+        if ((int)(s.block.x * s.block.y * s.block.z) > attr.maxThreadsPerBlock) return false;
+        if (s.lds + attr.sharedSizeBytes > kLdsPerWorkgroup) return false;
+    }
+    return true;
+}
+
 void plan_materialize_external(Plan &p, const std::vector<const int64_t *> &dev_tables)
 {
     // row tables already on the device (caller-owned, e.g. uploaded asynchronously)
@@ -984,9 +1019,15 @@ void plan_materialize(Plan &p)
     SIP_HIP_CHECK(hipGetDevice(&p.device));
     if (!p.external_tables.empty()) {
         for (Stage &s : p.stages) {
-            if (s.kind != StageKind::Stockham) continue;
-            if (s.in_tab >= 0) s.sa.tin.tab = p.external_tables[(size_t)s.in_tab];
-            if (s.out_tab >= 0) s.sa.tout.tab = p.external_tables[(size_t)s.out_tab];
+            const int64_t *ti = s.in_tab >= 0 ? p.external_tables[(size_t)s.in_tab] : nullptr;
+            const int64_t *to = s.out_tab >= 0 ? p.external_tables[(size_t)s.out_tab] : nullptr;
+            if (s.kind == StageKind::Stockham) {
+                if (ti) s.sa.tin.tab = ti;
+                if (to) s.sa.tout.tab = to;
+            } else {
+                if (ti) s.xa.rin.tab = ti;
+                if (to) s.xa.rout.tab = to;
+            }
         }
     } else if (p.row_tables_dev.size() != p.row_tables.size()) {
         for (void *d : p.row_tables_dev)
@@ -1026,6 +1067,10 @@ void plan_materialize(Plan &p)
             if (s.lds > 64 * 1024)
                 SIP_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s.lds));
         } else {
+            if (p.external_tables.empty()) {
+                if (s.in_tab >= 0) s.xa.rin.tab = static_cast<const int64_t *>(p.row_tables_dev[(size_t)s.in_tab]);
+                if (s.out_tab >= 0) s.xa.rout.tab = static_cast<const int64_t *>(p.row_tables_dev[(size_t)s.out_tab]);
+            }
             if (s.tab0.used()) {
                 s.tab0.dev = coeff_get(s.tab0);
                 s.xa.tab0 = static_cast<const float2 *>(s.tab0.dev->ptr);
@@ -1124,8 +1169,6 @@ std::string plan_describe(const Plan &p)
         }
         os << "}";
     }
-    os << "],\"reference_steps\":[";
-    for (size_t i = 0; i < p.ref_steps.size(); i++) os << (i ? "," : "") << "\"" << p.ref_steps[i] << "\"";
     os << "]}";
     return os.str();
 }

@@ -48,6 +48,8 @@ struct TableRef {
 // (counterpart of FFTensorCache::getCoeff, core/fft/utils/fftensor_cache.cpp:80-92).
 TablePtr coeff_get(const TableRef &ref);
 size_t coeff_cache_live_tables();
+// Map entries (live or expired), live tables and their bytes, over every device.
+void coeff_cache_stats(size_t *entries, size_t *live, size_t *live_bytes);
 
 enum BufKind : int { BUF_IN = 0, BUF_OUT = 1, BUF_WS = 2, BUF_ABS = 3 };
 
@@ -80,7 +82,7 @@ struct Stage {
     // fused 2D tile stage (both axes of X x Y slices in one launch); tw2 = column twiddles
     const Stockham2dInstance *inst2d = nullptr;
     TableRef tw2;
-    int in_tab = -1, out_tab = -1;  // row tables (Plan::row_tables) gathered by the in / out tiles
+    int in_tab = -1, out_tab = -1;  // row tables (Plan::row_tables) gathered by the in / out side
     // Aux
     AuxArgs xa{};
     TableRef tab0;
@@ -104,7 +106,6 @@ struct Plan {
     std::vector<size_t> ws_offsets;  // byte offset of each workspace region
     size_t ws_bytes = 0;
 
-    std::vector<std::string> ref_steps;  // the reference's step list for the same call
     std::mutex mu;
 
     // Gathered-row plans (batching layer): absolute base pointers and row-offset tables.
@@ -127,8 +128,9 @@ struct Plan {
 
 // 1D plan over rows gathered from arbitrary places: row r of the input starts at element
 // in_rows[r] of in_base, row r of the output at out_rows[r] of out_base (elements of each buffer's
-// own type: complex for C2C and R2C outputs, float for R2C inputs). C2C: any length; R2C: lengths
-// whose (half) transform is one leaf launch (gathered_supported).
+// own type: complex for C2C and R2C outputs, float for R2C inputs). C2C: any length (auxiliary
+// stages such as Bluestein's chirp steps and the length-1 copy read and write through the same row
+// tables); R2C: lengths whose (half) transform is one leaf launch (gathered_supported).
 void plan_build_gathered(Plan &p, int type, int64_t n, const std::vector<int64_t> &in_rows,
                          const std::vector<int64_t> &out_rows, void *in_base, void *out_base);
 void plan_build_gathered_c2c(Plan &p, int64_t n, const std::vector<int64_t> &in_rows,
@@ -141,6 +143,9 @@ void plan_materialize_external(Plan &p, const std::vector<const int64_t *> &dev_
 void plan_build(Plan &p);
 // Creates the device-side coefficient tables and kernel attributes. Throws SipError.
 void plan_materialize(Plan &p);
+// True when every stage's launch shape (threads, static + dynamic LDS) is within what the device
+// accepts for its kernel; checked before a tuning candidate is materialized or launched.
+bool plan_launchable(const Plan &p);
 // Enqueues the stages on p.stream. in/out are device addresses of the user tensors.
 void plan_execute(Plan &p, const void *in, void *out);
 // Split-plane (C2C_SEP) execution: user buffers are real / imaginary float planes.
@@ -161,12 +166,5 @@ int64_t instance_count();
 
 // Launchers implemented in the .hip files.
 hipError_t aux_launch(const AuxArgs &a, hipStream_t stream);
-
-// The reference's core decomposition for the same call (restated rules of
-// core/fft/fft_api.cpp:169-245, 383-697 and core/fft/fftcore/select_core.cpp:155-200).
-std::vector<std::string> reference_steps(int fftType, int direction, const std::vector<int64_t> &sizes,
-                                         int64_t batch, int dimType);
-// The reference's steps of an ISTFT plan (core/fft/istft.cpp:91-104).
-std::vector<std::string> reference_istft_steps(int64_t channel, int64_t nfft, int64_t frames, int64_t hop);
 
 }  // namespace sip

@@ -41,6 +41,8 @@ double now_s()
     return std::chrono::duration<double>(clk::now().time_since_epoch()).count();
 }
 
+size_t granule(size_t bytes) { return bytes == 0 ? 256 : (bytes + 255) / 256 * 256; }
+
 // First-fit allocator over one device arena (offsets; 256-byte granules).
 class Arena {
 public:
@@ -97,6 +99,7 @@ struct Req {
     void *out = nullptr;
     bool own_out = false;
     size_t out_off = SIZE_MAX, out_bytes = 0;
+    size_t ws_need = 0;  // workspace of the request's plan when it runs alone
     double t_arr = 0, t_adm = 0, t_done = 0;
     bool done = false;
     int32_t status = 0;
@@ -160,9 +163,21 @@ public:
         r->out = out;
         r->own_out = out == nullptr;
         r->out_bytes = (size_t)batch * (size_t)(type == kTypeC2C ? n : n / 2 + 1) * 8;
-        // a request that could never be admitted alone is refused (its launch needs its own output,
-        // a workspace of at most one complex copy, and the row tables)
-        const size_t need = (r->own_out ? r->out_bytes : 0) + (size_t)batch * (size_t)n * 8 + (size_t)batch * 32 + 4096;
+        // A request that could never be admitted alone is refused. Run alone, its launch needs its
+        // own output (when the server owns it) and the workspace of its plain plan; the layout is
+        // host-only work of a few microseconds.
+        try {
+            Plan dry;
+            dry.fftType = type;
+            dry.direction = kDirForward;
+            dry.sizes = {n};
+            dry.batch = batch;
+            plan_build(dry);
+            r->ws_need = dry.ws_bytes;
+        } catch (const SipError &e) {
+            return e.status;
+        }
+        const size_t need = (r->own_out ? granule(r->out_bytes) : 0) + (r->ws_need ? granule(r->ws_need) : 0);
         if (need > budget_) return AsdSip::ACL_ERROR_INVALID_PARAM;
         r->t_arr = now_s();
         std::lock_guard<std::mutex> lk(mu_);
@@ -351,7 +366,26 @@ private:
                 rows += r->batch;
                 if (!gathered) break;
             }
-            if (!launch_locked(group, rows, sidx)) break;  // out of memory: wait for completions
+            bool ok = launch_locked(group, rows, sidx);
+            if (!ok && group.size() > 1) {
+                // the coalesced launch does not fit now: try the head request alone
+                group.resize(1);
+                rows = head->batch;
+                ok = launch_locked(group, rows, sidx);
+            }
+            if (!ok) {
+                if (!inflight_.empty()) break;  // out of memory: wait for completions to free some
+                // Nothing runs, so the arena is empty and the head request can never fit (submit
+                // admits only requests whose lone launch fits the budget; this catches the rest):
+                // it fails instead of blocking the queue.
+                head->done = true;
+                head->status = AsdSip::ACL_ERROR_INTERNAL_ERROR;
+                head->t_done = now_s();
+                queue_.pop_front();
+                done_cv_.notify_all();
+                any = true;
+                continue;
+            }
             for (Req *r : group) queue_.erase(std::find(queue_.begin(), queue_.end(), r));
             any = true;
         }
@@ -442,8 +476,39 @@ private:
                 plan_build(*plan);
                 plan_materialize(*plan);
             }
+            keep_resident(type, n, *plan);
+            if (plan->ws_bytes) {
+                l->ws_bytes = plan->ws_bytes;
+                l->ws_off = arena_->alloc(l->ws_bytes);
+                if (l->ws_off == SIZE_MAX) {
+                    rollback();
+                    return false;
+                }
+                plan->workspace = static_cast<char *>(arena_base_) + l->ws_off;
+            }
+            plan->stream = streams_[(size_t)sidx];
+            const double t = now_s();
+            for (Req *r : group) r->t_adm = t;
+            if (gathered)
+                plan_execute(*plan, nullptr, nullptr);
+            else
+                plan_execute(*plan, h->in, h->out);
+            if (!ev_pool_.empty()) {
+                l->ev = ev_pool_.back();
+                ev_pool_.pop_back();
+            } else {
+                SIP_HIP_CHECK(hipEventCreateWithFlags(&l->ev, hipEventDisableTiming));
+            }
+            const hipError_t er = hipEventRecord(l->ev, streams_[(size_t)sidx]);
+            if (er != hipSuccess) {
+                ev_pool_.push_back(l->ev);
+                l->ev = nullptr;
+                SIP_HIP_CHECK(er);
+            }
         } catch (...) {
-            // the requests fail; the caller removes them from the queue
+            // The requests fail; the caller removes them from the queue. Work that was already
+            // enqueued (a table copy, the first stages) finishes before its memory is reused.
+            (void)hipStreamSynchronize(streams_[(size_t)sidx]);
             rollback();
             for (Req *r : group) {
                 r->done = true;
@@ -453,30 +518,6 @@ private:
             done_cv_.notify_all();
             return true;
         }
-        keep_resident(type, n, *plan);
-        if (plan->ws_bytes) {
-            l->ws_bytes = plan->ws_bytes;
-            l->ws_off = arena_->alloc(l->ws_bytes);
-            if (l->ws_off == SIZE_MAX) {
-                rollback();
-                return false;
-            }
-            plan->workspace = static_cast<char *>(arena_base_) + l->ws_off;
-        }
-        plan->stream = streams_[(size_t)sidx];
-        const double t = now_s();
-        for (Req *r : group) r->t_adm = t;
-        if (gathered)
-            plan_execute(*plan, nullptr, nullptr);
-        else
-            plan_execute(*plan, h->in, h->out);
-        if (!ev_pool_.empty()) {
-            l->ev = ev_pool_.back();
-            ev_pool_.pop_back();
-        } else {
-            SIP_HIP_CHECK(hipEventCreateWithFlags(&l->ev, hipEventDisableTiming));
-        }
-        SIP_HIP_CHECK(hipEventRecord(l->ev, streams_[(size_t)sidx]));
         l->reqs = group;
         l->plan = std::move(plan);
         inflight_per_stream_[(size_t)sidx]++;
@@ -644,6 +685,8 @@ int32_t sipFftServeBenchmark(void *handle, double rate, double seconds, uint64_t
             return E::ACL_ERROR_INVALID_PARAM;
         const std::vector<int64_t> sm = sip::smooth_between(n_lo, n_hi);
         if (sm.empty()) return E::ACL_ERROR_INVALID_PARAM;
+        // every request's input (one row at least) must fit in the pool
+        if (pool_bytes < (size_t)sm.back() * 8) return E::ACL_ERROR_INVALID_PARAM;
         std::mt19937_64 rng(seed);
         std::exponential_distribution<double> gap(rate);
         std::uniform_real_distribution<double> u01(0.0, 1.0);
@@ -667,11 +710,10 @@ int32_t sipFftServeBenchmark(void *handle, double rate, double seconds, uint64_t
             const double x = std::exp(lo + (hi - lo) * u01(rng));
             g.n = *std::lower_bound(sm.begin(), sm.end(), (int64_t)std::ceil(x));
             g.b = ub(rng);
-            const size_t need = (size_t)g.b * (size_t)g.n * (g.type == sip::kTypeC2C ? 8 : 4);
-            if (need > pool_bytes) {
-                g.b = std::max<int64_t>(1, (int64_t)(pool_bytes / ((size_t)g.n * 8)));
-            }
-            const size_t need2 = (size_t)g.b * (size_t)g.n * (g.type == sip::kTypeC2C ? 8 : 4);
+            const size_t esz = g.type == sip::kTypeC2C ? 8 : 4;
+            if ((size_t)g.b * (size_t)g.n * esz > pool_bytes)
+                g.b = std::max<int64_t>(1, (int64_t)(pool_bytes / ((size_t)g.n * esz)));
+            const size_t need2 = (size_t)g.b * (size_t)g.n * esz;  // <= pool_bytes (checked above)
             const size_t span = pool_bytes - need2;
             g.off = (size_t)(u01(rng) * (double)(span / 8)) * 8;
             gen.push_back(g);

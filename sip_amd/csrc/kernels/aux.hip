@@ -14,7 +14,12 @@ __device__ __forceinline__ int64_t row_base(uint32_t r, const AuxRows &d)
 {
     const uint32_t r1 = fdiv_aux(r, d.div_n0);
     const uint32_t r0 = r - r1 * (uint32_t)d.n0;
-    return (int64_t)r0 * d.s0 + (int64_t)r1 * d.s1;
+    int64_t o0 = (int64_t)r0 * d.s0, o1 = (int64_t)r1 * d.s1;
+    if (d.tab) {
+        if (d.tab_dim == 0) o0 = d.tab[r0];
+        else o1 = d.tab[r1];
+    }
+    return o0 + o1;
 }
 
 // Complex element at offset o of the input / output, interleaved or split into two planes.

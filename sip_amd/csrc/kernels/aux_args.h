@@ -26,6 +26,11 @@ struct AuxRows {
     int64_t s0, s1;    // strides of r0, r1 (elements)
     int64_t es;        // element stride
     FastDiv div_n0;
+    // Gathered rows: when tab != null, the offset contributed by index r<tab_dim> is tab[r<tab_dim>]
+    // (elements from the launch's base pointer) instead of r<tab_dim> * s<tab_dim>.
+    const int64_t *tab;
+    int32_t tab_dim;
+    int32_t pad_;
 };
 
 struct AuxArgs {

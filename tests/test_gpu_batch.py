@@ -128,3 +128,103 @@ def test_autotune_leaves_no_pending_device_error(torch_cuda, tmp_path, monkeypat
         rel, _ = report(f"tuned problem {lengths[i]}", ys[i].cpu().numpy(), oracle.c2c(hs[i]))
         assert rel <= TOL
     bh.destroy()
+
+
+def test_gathered_aux_stages(torch_cuda):
+    """Lengths whose plans carry auxiliary stages over the gathered rows: 1 (a copy), 17, 997 and 6007
+    (Bluestein chirp, kernel and post stages) and 20000 (four-step); problems interleaved in memory
+    so that a stage reading or writing the wrong row shows up."""
+    torch = torch_cuda
+    lengths = [17, 1, 997, 17, 1, 6007, 997, 20000, 17]
+    batches = [3, 5, 2, 4, 2, 1, 3, 2, 1]
+    hs, xs, ys = _problems(torch, lengths, batches, 1300)
+    for direction in (FWD, INV):
+        for y in ys:
+            y.fill_(float("nan"))
+        bh = _run_batch(torch, lengths, batches, xs, ys, direction, S.BATCH_GRAPH)
+        sgn = oracle.FORWARD if direction == FWD else oracle.INVERSE
+        for i, (n, b) in enumerate(zip(lengths, batches)):
+            rel, _ = report(f"gathered aux n={n} b={b} dir={direction:#x}", ys[i].cpu().numpy(),
+                            oracle.c2c(hs[i], sgn))
+            assert rel <= (5e-6 if n in (17, 997, 6007) else TOL)
+        bh.destroy()
+
+
+def test_autotune_in_place(torch_cuda, tmp_path, monkeypatch):
+    """In-place problems (output buffer = input buffer) with autotuning: the tuner times its
+    candidates on scratch rows, so the caller's data is transformed exactly once, by the exec."""
+    torch = torch_cuda
+    monkeypatch.setenv("SIP_FFT_TUNE_CACHE", str(tmp_path / "tune.jsonl"))
+    lengths = [4096, 1000, 65536, 997, 1536]
+    batches = [8, 6, 2, 3, 5]
+    hs, xs, _ = _problems(torch, lengths, batches, 1500)
+    bh = _run_batch(torch, lengths, batches, xs, xs, FWD, S.BATCH_GRAPH | S.BATCH_AUTOTUNE)
+    for i, (n, b) in enumerate(zip(lengths, batches)):
+        rel, _ = report(f"in-place tuned n={n}", xs[i].cpu().numpy(), oracle.c2c(hs[i]))
+        assert rel <= (5e-6 if n == 997 else TOL)
+    bh.destroy()
+
+
+def test_batching_config_workload(torch_cuda, tmp_path, monkeypatch):
+    """The batching configuration itself: 4096 C2C problems, lengths log-uniform over 2^a 3^b 5^c in
+    [64, 65536], batch U[1, 64] (257 distinct lengths, 131,586 transforms), inputs uniform[-1, 1).
+    The autotuned hipGraph launch is bit-identical to the same tuned plans launched directly, and
+    sampled rows of every length (the first and the last row of each length group, plus one random
+    problem's rows) are within 2e-6 of the FP64 oracle for the graph and for the untuned grouped
+    launches."""
+    torch = torch_cuda
+    monkeypatch.setenv("SIP_FFT_TUNE_CACHE", str(tmp_path / "tune.jsonl"))
+    lengths, batches = S.batching_workload(4096)
+    elems = [n * b for n, b in zip(lengths, batches)]
+    offs = np.concatenate([[0], np.cumsum(elems)]).astype(np.int64)
+    total = int(offs[-1])
+    print(f"batching workload: {len(lengths)} problems, {len(set(lengths))} lengths, {total} points", flush=True)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0x5EED0004)
+    x = torch.view_as_complex(torch.rand((total, 2), generator=g, device="cuda") * 2 - 1)
+    y_graph = torch.empty_like(x)
+    y_direct = torch.empty_like(x)
+    y_plain = torch.empty_like(x)
+    base = x.data_ptr()
+    ins = [base + 8 * int(o) for o in offs[:-1]]
+
+    def outs(t):
+        return [t.data_ptr() + 8 * int(o) for o in offs[:-1]]
+
+    s = torch.cuda.current_stream().cuda_stream
+    handles = []
+    for y, flags in ((y_graph, S.BATCH_GRAPH | S.BATCH_AUTOTUNE), (y_direct, S.BATCH_AUTOTUNE), (y_plain, 0)):
+        bh = S.BatchHandle()
+        assert bh.make_plan(lengths, batches, ins, outs(y), FWD, flags) == 0
+        bh.alloc_workspace()
+        if s:
+            assert bh.set_stream(s) == 0
+        assert bh.exec() == 0
+        handles.append(bh)
+    torch.cuda.synchronize()
+    d = handles[0].describe()
+    assert d["groups"] == len(set(lengths)) and d["graph"] == 1
+    assert torch.equal(y_graph, y_direct), "graph launch differs from the direct launches of the same plans"
+    # sampled rows: first and last row of every length group, and all rows of one random problem
+    first, last = {}, {}
+    for i, n in enumerate(lengths):
+        first.setdefault(n, i)
+        last[n] = i
+    rows = []
+    for n in sorted(first):
+        rows.append((int(offs[first[n]]), n))
+        i = last[n]
+        rows.append((int(offs[i]) + (batches[i] - 1) * n, n))
+    pick = int(np.random.default_rng(5).integers(0, len(lengths)))
+    rows += [(int(offs[pick]) + r * lengths[pick], lengths[pick]) for r in range(batches[pick])]
+    worst = 0.0
+    for start, n in rows:
+        xr = x[start:start + n].cpu().numpy().reshape(1, n)
+        ref = oracle.c2c(xr)
+        for name, y in (("graph", y_graph), ("plain grouped", y_plain)):
+            rel, _ = oracle.error(y[start:start + n].cpu().numpy().reshape(1, n), ref)
+            worst = max(worst, rel)
+            assert rel <= TOL, f"{name} row at {start} (n={n}): rel-L2 {rel:.3e}"
+    print(f"[parity] batching workload: {len(rows)} sampled rows, worst rel-L2 {worst:.3e}", flush=True)
+    for bh in handles:
+        bh.destroy()

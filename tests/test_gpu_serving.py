@@ -97,11 +97,162 @@ def test_benchmark_modest_rate(torch_cuda):
 
 
 def test_submit_validation(torch_cuda):
-    srv = S.Server(budget_bytes=1 << 26, nstreams=1, max_coalesce_rows=64)
+    """Requests refused at submit: C2R (not served), n = 0, an output beyond the budget, and a Bluestein
+    length whose workspace (86.6 MB for 10007 x 100) exceeds the 64 MiB budget although its
+    input and output would fit. The input is a real device allocation, so a request that passed
+    validation by mistake would still run in bounds."""
+    torch = torch_cuda
     import ctypes
+    x = torch.zeros((100, 10007), dtype=torch.complex64, device="cuda")
+    y = torch.empty_like(x)
+    srv = S.Server(budget_bytes=1 << 26, nstreams=1, max_coalesce_rows=64)
     t = ctypes.c_uint64()
     L = S.lib()
-    assert L.sipFftServeSubmit(srv.h, 0x11, 1024, 1, ctypes.c_void_p(0x1000), None, ctypes.byref(t)) != 0  # C2R
-    assert L.sipFftServeSubmit(srv.h, 0x10, 0, 1, ctypes.c_void_p(0x1000), None, ctypes.byref(t)) != 0
-    assert L.sipFftServeSubmit(srv.h, 0x10, 1 << 20, 512, ctypes.c_void_p(0x1000), None, ctypes.byref(t)) != 0  # > budget
+    xp = ctypes.c_void_p(x.data_ptr())
+    assert L.sipFftServeSubmit(srv.h, 0x11, 1024, 1, xp, None, ctypes.byref(t)) != 0  # C2R
+    assert L.sipFftServeSubmit(srv.h, 0x10, 0, 1, xp, None, ctypes.byref(t)) != 0
+    assert L.sipFftServeSubmit(srv.h, 0x10, 10007, 100, xp, None, ctypes.byref(t)) != 0   # output > budget
+    assert L.sipFftServeSubmit(srv.h, 0x10, 10007, 100, xp, ctypes.c_void_p(y.data_ptr()),
+                               ctypes.byref(t)) != 0                                     # workspace > budget
+    # the same length at a batch whose workspace fits is served and correct
+    tk = srv.submit(S.ASCEND_FFT_C2C, 10007, 10, x.data_ptr(), y.data_ptr())
+    st, _ = srv.wait(tk)
+    assert st == 0
+    srv.destroy()
+
+
+def _watched_inputs(torch, t, n, b, seed):
+    if t == S.ASCEND_FFT_C2C:
+        x = oracle.uniform_complex((b, n), seed)
+        y = torch.empty((b, n), dtype=torch.complex64, device="cuda")
+    else:
+        x = oracle.uniform(b * n, seed).reshape(b, n)
+        y = torch.empty((b, n // 2 + 1), dtype=torch.complex64, device="cuda")
+    return x, torch.from_numpy(x).cuda(), y
+
+
+def _check_rows(tag, t, n, x, y):
+    """Oracle check of the first and last row of a watched request."""
+    rows = sorted({0, x.shape[0] - 1})
+    ref = oracle.c2c(x[rows]) if t == S.ASCEND_FFT_C2C else oracle.r2c(x[rows])
+    rel, _ = report(tag, y.cpu().numpy()[rows], ref)
+    assert rel <= 2e-6
+
+
+def test_poisson_stream_2k_results(torch_cuda):
+    """A seeded Poisson stream at 2,000 requests/s for one second, drawn as the serving configuration
+    draws it (70 % C2C / 30 % R2C, n log-uniform over the 2,3,5-smooth lengths in [256, 32768], batch
+    U[1, 512], inputs uniform[-1, 1) slices of a device pool). Every 50th request writes to an output
+    of its own: each of those is bit-identical to the same request run alone, and its first and last
+    rows are within 2e-6 of the FP64 oracle. Every request completes."""
+    import time
+    torch = torch_cuda
+    rng = np.random.Generator(np.random.PCG64(2000))
+    sm = S.smooth_lengths(256, 32768)
+    pool = torch.rand(256 << 20, device="cuda") * 2 - 1          # 1 GiB of uniform data
+    srv = S.Server(budget_bytes=64 << 30, nstreams=4, max_coalesce_rows=8192)
+    rate, seconds = 2000.0, 1.0
+    reqs, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= seconds:
+            break
+        ty = S.ASCEND_FFT_C2C if rng.random() < 0.7 else S.ASCEND_FFT_R2C
+        x = np.exp(rng.uniform(np.log(256), np.log(32768)))
+        n = next(v for v in sm if v >= x)
+        b = int(rng.integers(1, 513))
+        reqs.append((t, ty, n, b))
+    watched = {}
+    for k in range(0, len(reqs), 50):
+        _, ty, n, b = reqs[k]
+        watched[k] = _watched_inputs(torch, ty, n, b, 7000 + k)
+    torch.cuda.synchronize()
+    tickets = []
+    t0 = time.perf_counter()
+    for k, (due, ty, n, b) in enumerate(reqs):
+        while time.perf_counter() - t0 < due:
+            pass
+        if k in watched:
+            _, xd, yd = watched[k]
+            tickets.append(srv.submit(ty, n, b, xd.data_ptr(), yd.data_ptr()))
+        else:
+            esz = 8 if ty == S.ASCEND_FFT_C2C else 4
+            span = pool.numel() * 4 - n * b * esz
+            off = int(rng.integers(0, max(1, span // 8))) * 8
+            tickets.append(srv.submit(ty, n, b, pool.data_ptr() + off))
+    for tk in tickets:
+        st, lat = srv.wait(tk)
+        assert st == 0 and lat > 0
+    stats = srv.stats()
+    print(f"serving 2k/s: {len(reqs)} requests, p50 {stats['p50_ms']:.3f} ms, p99 {stats['p99_ms']:.3f} ms, "
+          f"{stats['launches']} launches", flush=True)
+    assert stats["completed"] == len(reqs)
+    for k, (x, xd, yd) in watched.items():
+        _, ty, n, b = reqs[k]
+        alone = _alone(torch, ty, n, b, xd)
+        assert torch.equal(alone, yd), f"request {k} (type {ty:#x}, n={n}, b={b}) differs alone vs in the stream"
+        _check_rows(f"serving 2k request {k} n={n} b={b}", ty, n, x, yd)
+    srv.destroy()
+
+
+def test_admission_waits_for_memory(torch_cuda):
+    """A 4 GiB budget and a burst of 28 requests that need 512 MiB each, either of server-owned
+    output (C2C 16384 x 4096) or of four-step workspace (C2C 65536 x 1024, caller's output): at most
+    eight fit at a time while 16 streams offer 32 launch slots, so memory is what binds and later
+    requests wait in the queue until completions free arena space. Every request completes, the
+    arena peaks at the budget and never beyond, none is dropped, and the watched results equal the
+    requests run alone."""
+    torch = torch_cuda
+    budget = 4 << 30
+    need = 512 << 20
+    srv = S.Server(budget_bytes=budget, nstreams=16, max_coalesce_rows=1024)
+    pool = torch.rand(128 << 20, device="cuda") * 2 - 1                    # 512 MiB: one C2C 16384 x 4096
+    watched, tickets = [], []
+    for k in range(4):
+        x, xd, yd = _watched_inputs(torch, S.ASCEND_FFT_C2C, 65536, 1024, 8100 + k)
+        watched.append((x, xd, yd))
+    torch.cuda.synchronize()
+    for k in range(24):
+        tickets.append(srv.submit(S.ASCEND_FFT_C2C, 16384, 4096, pool.data_ptr()))
+        if k % 6 == 5:
+            x, xd, yd = watched[k // 6]
+            tickets.append(srv.submit(S.ASCEND_FFT_C2C, 65536, 1024, xd.data_ptr(), yd.data_ptr()))
+    for tk in tickets:
+        st, _ = srv.wait(tk)
+        assert st == 0
+    stats = srv.stats()
+    print(f"admission: peak {stats['arena_peak'] / 2**20:.0f} MiB of {budget / 2**20:.0f}, "
+          f"{stats['launches']} launches", flush=True)
+    assert stats["completed"] == 28 and stats["launches"] == 28
+    assert budget - need < stats["arena_peak"] <= budget, stats
+    for k, (x, xd, yd) in enumerate(watched):
+        alone = _alone(torch, S.ASCEND_FFT_C2C, 65536, 1024, xd)
+        assert torch.equal(alone, yd), f"watched request {k} differs alone vs under admission pressure"
+        _check_rows(f"admission watched {k}", S.ASCEND_FFT_C2C, 65536, x, yd)
+    srv.destroy()
+
+
+def test_coalesced_aux_lengths(torch_cuda):
+    """Coalesced launches of lengths whose plans carry auxiliary stages over the gathered rows
+    (1, 17 and 997: copy and Bluestein; 20000: four-step): every result equals the request alone."""
+    torch = torch_cuda
+    srv = S.Server(budget_bytes=4 << 30, nstreams=1, max_coalesce_rows=4096)
+    cases = []
+    for k, n in enumerate([17, 997, 1, 17, 997, 1, 20000, 20000, 17]):
+        x, xd, yd = _watched_inputs(torch, S.ASCEND_FFT_C2C, n, 3 + k % 4, 8300 + k)
+        cases.append((n, x, xd, yd))
+    torch.cuda.synchronize()
+    # a long request first keeps the single stream busy so the rest queue up and coalesce
+    big = torch.zeros((64, 65536), dtype=torch.complex64, device="cuda")
+    tickets = [srv.submit(S.ASCEND_FFT_C2C, 65536, 64, big.data_ptr())]
+    for n, x, xd, yd in cases:
+        tickets.append(srv.submit(S.ASCEND_FFT_C2C, n, x.shape[0], xd.data_ptr(), yd.data_ptr()))
+    for tk in tickets:
+        st, _ = srv.wait(tk)
+        assert st == 0
+    for n, x, xd, yd in cases:
+        alone = _alone(torch, S.ASCEND_FFT_C2C, n, x.shape[0], xd)
+        assert torch.equal(alone, yd), f"n={n}: coalesced result differs from the request alone"
+        rel, _ = report(f"coalesced aux n={n}", yd.cpu().numpy(), oracle.c2c(x))
+        assert rel <= (5e-6 if n in (17, 997) else 2e-6)
     srv.destroy()

@@ -12,6 +12,7 @@ import threading
 
 import pytest
 
+import oracle
 import sip_amd as S
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -113,6 +114,27 @@ def test_istft_make_plan_validation():
     h.destroy()
 
 
+def test_istft_make_plan_failure_resets_the_handle():
+    """A valid ISTFT request whose plan cannot be built (here: no GPU to hold its tables) returns
+    500000, and the handle keeps no ISTFT description, so exec answers 100000."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present: the plan builds")
+    except ImportError:
+        pass
+    L = S.lib()
+    h = S.FftHandle()
+    x = S.AclTensor(0x1000, [2, 16, 5], S.ACL_COMPLEX64)
+    w = S.AclTensor(0x2000, [16], S.ACL_FLOAT)
+    y = S.AclTensor(0x3000, [2, 16 + 4 * 4 - 16], S.ACL_COMPLEX64)
+    assert L.asdFftIstftMakePlan(h.h, x.ptr, 16, 4, 16, True, False, False, 0, True) == INTERNAL
+    assert L.asdFftExecIstft(h.h, x.ptr, w.ptr, y.ptr) == INV
+    for t in (x, w, y):
+        t.close()
+    h.destroy()
+
+
 def test_acl_tensor_descriptor():
     L = S.lib()
     t = S.AclTensor(0x2000, [3, 5, 7], S.ACL_FLOAT)
@@ -151,16 +173,28 @@ def dry(dims, t, batch, dim_type=S.ASCEND_FFT_HORIZONTAL, direction=FWD):
     ([64, 64], S.ASCEND_FFT_C2C, 2, ["Dd(X=64,Y=64,batch=2)"]),
 ])
 def test_reference_core_selection(dims, t, batch, expect):
-    d = dry(dims, t, batch)
+    steps = oracle.reference_steps(dims, t, batch)
     if expect is None:
-        assert d["reference_steps"][0].startswith("FftMix(N=15000")
+        assert steps[0].startswith("FftMix(N=15000")
     else:
-        assert d["reference_steps"] == expect
+        assert steps == expect
 
 
 def test_vertical_reference_rule():
-    d = dry([4096], S.ASCEND_FFT_C2C, 128, S.ASCEND_FFT_VERTICAL)
-    assert d["reference_steps"] == ["FftStride(N=4096,batch=128,stride=128)"]
+    steps = oracle.reference_steps([4096], S.ASCEND_FFT_C2C, 128, S.ASCEND_FFT_VERTICAL)
+    assert steps == ["FftStride(N=4096,batch=128,stride=128)"]
+
+
+def test_reference_istft_steps():
+    assert oracle.reference_istft_steps(2, 400, 11, 100) == [
+        "Transpose(1,2,dims=[2,400,11])", "FftMix(N=400,batch=22,radix=[20,20])",
+        "IstftAny(n_fft=400,hop=100,frames=11,channel=2)"]
+
+
+def test_product_library_holds_no_reference_rules():
+    """The restated reference rules are test infrastructure: libsip_fft.so's plan description
+    carries only this library's own stages."""
+    assert "reference_steps" not in dry([4096], S.ASCEND_FFT_C2C, 16384)
 
 
 def test_plan_layouts():
