@@ -194,6 +194,7 @@ def bench_precision(torch, S, stream, steps):
                 g = torch.Generator(device="cuda")
                 g.manual_seed(0x5EED0005)
                 xi = torch.view_as_complex(torch.rand([b] + dims + [2], generator=g, device="cuda") * 2 - 1).contiguous()
+            torch.cuda.synchronize()  # xi was written on `stream`; the host copy below runs on another
             rows = np.arange(0, b, b // 64)
             xs = xi.cpu().numpy()[rows]
             ref = oracle.c2c(xs)

@@ -11,6 +11,44 @@ namespace sip {
 template <int... R>
 struct Radices {};
 
+// Element u of thread tid in HBM order over an X x Y slice (Y contiguous): i = tid + u NT, x = i / Y,
+// y = i mod Y. When NT is a multiple of Y and divides X Y (every instance: powers of two), x = x0 +
+// u DX with DX = NT / Y and x0 = tid / Y < DX, y = tid mod Y, so the padded LDS indices below are one
+// per-thread base plus a compile-time offset per u (no per-u address registers live across the
+// radix passes).
+template <int X, int Y, int NT>
+struct SliceMap {
+    static constexpr int kPer = (X * Y + NT - 1) / NT;
+    static constexpr bool kExact = (NT % Y) == 0 && ((X * Y) % NT) == 0;
+    static constexpr int DX = kExact ? NT / Y : 1;
+    static_assert(!kExact || DX % 16 == 0 || 16 % DX == 0, "DX must be a power of two");
+    int tid, x0, y0;
+    __device__ __forceinline__ explicit SliceMap(int t) : tid(t), x0(t / Y), y0(t - (t / Y) * Y) {}
+    __device__ __forceinline__ bool ok(int u) const { return kExact || tid + u * NT < X * Y; }
+    // [x][y] layout with padded pitch P
+    template <int P>
+    __device__ __forceinline__ int xy(int u) const
+    {
+        if constexpr (kExact) {
+            return x0 * P + lds_pad(y0) + u * DX * P;
+        } else {
+            const int i = tid + u * NT, x = i / Y;
+            return x * P + lds_pad(i - x * Y);
+        }
+    }
+    // [y][x] layout with padded pitch Q: pad(x0 + u DX) = pad(x0) + u DX + (u DX) / 16 for x0 < DX
+    template <int Q>
+    __device__ __forceinline__ int yx(int u) const
+    {
+        if constexpr (kExact) {
+            return y0 * Q + lds_pad(x0) + u * DX + ((u * DX) >> 4);
+        } else {
+            const int i = tid + u * NT, x = i / Y;
+            return (i - x * Y) * Q + lds_pad(x);
+        }
+    }
+};
+
 template <int X, int TX, int Y, int TY, int S, class RX, class RY>
 struct Tile2d;
 
@@ -29,21 +67,19 @@ struct Tile2d<X, TX, Y, TY, S, Radices<RX...>, Radices<RY...>> {
         const int64_t slice = blockIdx.x;
         const float2 *src = static_cast<const float2 *>(a.in) + slice * (int64_t)(X * Y);
         float2 *dst = static_cast<float2 *>(a.out) + slice * (int64_t)(X * Y);
+        const SliceMap<X, Y, kThreads> m(tid);
         // load [x][y]: consecutive lanes read consecutive elements, all loads in flight first
         {
             float2 v[kPer];
 #pragma unroll
-            for (int u = 0; u < kPer; u++) {
-                const int i = tid + u * kThreads;
-                if (i < X * Y) v[u] = src[i];
-            }
+            for (int u = 0; u < kPer; u++)
+                if (m.ok(u)) v[u] = (src + u * kThreads)[tid];
 #pragma unroll
             for (int u = 0; u < kPer; u++) {
-                const int i = tid + u * kThreads;
-                if (i < X * Y) {
-                    const int x = i / Y, y = i - x * Y;
-                    re[x * kLpy + lds_pad(y)] = v[u].x;
-                    im[x * kLpy + lds_pad(y)] = v[u].y;
+                if (m.ok(u)) {
+                    const int j = m.template xy<kLpy>(u);
+                    re[j] = v[u].x;
+                    im[j] = v[u].y;
                 }
             }
         }
@@ -60,20 +96,18 @@ struct Tile2d<X, TX, Y, TY, S, Radices<RX...>, Radices<RY...>> {
             float2 v[kPer];
 #pragma unroll
             for (int u = 0; u < kPer; u++) {
-                const int i = tid + u * kThreads;
-                if (i < X * Y) {
-                    const int x = i / Y, y = i - x * Y;
-                    v[u] = make_float2(re[x * kLpy + lds_pad(y)], im[x * kLpy + lds_pad(y)]);
+                if (m.ok(u)) {
+                    const int j = m.template xy<kLpy>(u);
+                    v[u] = make_float2(re[j], im[j]);
                 }
             }
             __syncthreads();
 #pragma unroll
             for (int u = 0; u < kPer; u++) {
-                const int i = tid + u * kThreads;
-                if (i < X * Y) {
-                    const int x = i / Y, y = i - x * Y;
-                    re[y * kLpx + lds_pad(x)] = v[u].x;
-                    im[y * kLpx + lds_pad(x)] = v[u].y;
+                if (m.ok(u)) {
+                    const int j = m.template yx<kLpx>(u);
+                    re[j] = v[u].x;
+                    im[j] = v[u].y;
                 }
             }
             __syncthreads();
@@ -88,10 +122,9 @@ struct Tile2d<X, TX, Y, TY, S, Radices<RX...>, Radices<RY...>> {
         // store (x, y) from [y][x]: consecutive lanes write consecutive y
 #pragma unroll
         for (int u = 0; u < kPer; u++) {
-            const int i = tid + u * kThreads;
-            if (i < X * Y) {
-                const int x = i / Y, y = i - x * Y;
-                dst[i] = make_float2(re[y * kLpx + lds_pad(x)], im[y * kLpx + lds_pad(x)]);
+            if (m.ok(u)) {
+                const int j = m.template yx<kLpx>(u);
+                (dst + u * kThreads)[tid] = make_float2(re[j], im[j]);
             }
         }
     }
@@ -141,25 +174,19 @@ struct Tile2dSplit<X, TX, Y, TY, S, Radices<RX...>, Radices<RY...>> {
         const int rbase = (ract ? xr : 0) * kLpy, cbase = (cact ? yc : 0) * kLpx;
 
         // HBM [x][y] -> first row pass: consecutive lanes read consecutive elements
+        const SliceMap<X, Y, kThreads> m(tid);
         float2 rin[NBY0][RY0];
         {
             float2 v[kPer];
 #pragma unroll
-            for (int u = 0; u < kPer; u++) {
-                const int i = tid + u * kThreads;
-                if (i < X * Y) v[u] = src[i];
-            }
+            for (int u = 0; u < kPer; u++)
+                if (m.ok(u)) v[u] = (src + u * kThreads)[tid];
 #pragma unroll
             for (int plane = 0; plane < 2; plane++) {
                 if (plane) __syncthreads();
 #pragma unroll
-                for (int u = 0; u < kPer; u++) {
-                    const int i = tid + u * kThreads;
-                    if (i < X * Y) {
-                        const int x = i / Y, y = i - x * Y;
-                        pl[x * kLpy + lds_pad(y)] = plane ? v[u].y : v[u].x;
-                    }
-                }
+                for (int u = 0; u < kPer; u++)
+                    if (m.ok(u)) pl[m.template xy<kLpy>(u)] = plane ? v[u].y : v[u].x;
                 __syncthreads();
 #pragma unroll
                 for (int q = 0; q < NBY0; q++) {
@@ -240,10 +267,8 @@ struct Tile2dSplit<X, TX, Y, TY, S, Radices<RX...>, Radices<RY...>> {
             __syncthreads();
 #pragma unroll
             for (int u = 0; u < kPer; u++) {
-                const int i = tid + u * kThreads;
-                if (i < X * Y) {
-                    const int x = i / Y, y = i - x * Y;
-                    const float val = pl[x * kLpy + lds_pad(y)];
+                if (m.ok(u)) {
+                    const float val = pl[m.template xy<kLpy>(u)];
                     if (plane)
                         v[u].y = val;
                     else
@@ -252,10 +277,8 @@ struct Tile2dSplit<X, TX, Y, TY, S, Radices<RX...>, Radices<RY...>> {
             }
         }
 #pragma unroll
-        for (int u = 0; u < kPer; u++) {
-            const int i = tid + u * kThreads;
-            if (i < X * Y) dst[i] = v[u];
-        }
+        for (int u = 0; u < kPer; u++)
+            if (m.ok(u)) (dst + u * kThreads)[tid] = v[u];
     }
 };
 
