@@ -85,6 +85,15 @@ int32_t sipFftPlanTableDry(int32_t ndim, const int64_t *dims, int32_t fftType, i
  * A table is shared by every live plan that uses it and freed with the last of them. */
 int32_t sipFftCoeffCacheStats(size_t *entries, size_t *live_tables, size_t *live_bytes);
 
+/* asdFftExecC2C without a caller workspace. No counterpart in the reference: it is for callers
+ * that cannot allocate (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc). Same checks and status codes as asdFftExecC2C; the plan's workspace comes from a
+ * per-device scratch array of 64 MiB held by the library (looked up on the first call on a device,
+ * kept until the process ends), so asdFftSetWorkspace is not needed, and a plan whose workspace
+ * (asdFftGetWorkspaceSize) exceeds 64 MiB is refused with 100000. The call allocates nothing. All
+ * calls on one GPU share that GPU's array: calls that overlap on one GPU race. */
+int32_t sipFftExecC2CNoAlloc(void *handle, const aclTensor *input, const aclTensor *output);
+
 /* Library version string. */
 const char *sipFftVersion(void);
 

@@ -54,7 +54,7 @@ EXPORTED_SYMBOLS = [
     "sipFftBatchSetWorkspace", "sipFftBatchExec", "sipFftBatchDescribe", "sipFftBatchDestroy",
     "sipFftServeCreate", "sipFftServeSubmit", "sipFftServeWait", "sipFftServeDrain", "sipFftServeStats",
     "sipFftServeBenchmark", "sipFftServeDestroy", "sipFftSetPrecision", "sipFftPlanTableDry",
-    "sipFftCoeffCacheStats",
+    "sipFftCoeffCacheStats", "sipFftExecC2CNoAlloc",
 ]
 
 PREC_FP32, PREC_F16_SPLIT, PREC_BF16_SPLIT, PREC_BF16_SPLIT3 = 0, 1, 2, 3
@@ -120,6 +120,7 @@ def lib():
         "sipFftSetPrecision": [vp, i32],
         "sipFftPlanTableDry": [i32, P(i64), i32, i32, i64, i32, i32, i32, P(ctypes.c_float), sz, P(sz)],
         "sipFftCoeffCacheStats": [P(sz), P(sz), P(sz)],
+        "sipFftExecC2CNoAlloc": [vp, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -263,6 +264,10 @@ class FftHandle:
 
     def exec_c2c(self, x, y):
         return self._exec(lib().asdFftExecC2C, x, y)
+
+    def exec_c2c_noalloc(self, x, y):
+        """sipFftExecC2CNoAlloc: C2C with the workspace taken from the library's per-device scratch."""
+        return self._exec(lib().sipFftExecC2CNoAlloc, x, y)
 
     def exec_r2c(self, x, y):
         return self._exec(lib().asdFftExecR2C, x, y)

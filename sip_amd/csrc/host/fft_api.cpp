@@ -15,6 +15,7 @@
 
 #include "aclnn/acl_meta.h"
 #include "plan.h"
+#include "scratch.h"
 #include "tensor.h"
 
 using sip::Plan;
@@ -280,6 +281,47 @@ int32_t asdFftExecR2C(void *handle, const aclTensor *input, const aclTensor *out
         if (!match(*p, input, ACL_FLOAT, false)) return E::ACL_ERROR_INVALID_PARAM;
         if (!match(*p, output, ACL_COMPLEX64, true)) return E::ACL_ERROR_INVALID_PARAM;
         return exec_common(*p, input, output);
+    });
+}
+
+// asdFftExecC2C without a caller workspace, for callers that cannot allocate (code under HIP graph
+// capture, serving loops, harnesses that forbid hipMalloc). No counterpart in the reference. The
+// plan's workspace (four-step intermediate, Bluestein rows) comes from the library's per-device
+// scratch array of sip::kScratchBytes (64 MiB), whose address is looked up on the first call on a
+// device; a plan needing more is refused with 100000. Nothing is allocated, and no memory is
+// cleared: every stage writes all of the workspace it later reads. Every call on a GPU shares that
+// GPU's array, so calls that overlap on one GPU (other streams, other threads) race.
+int32_t sipFftExecC2CNoAlloc(void *handle, const aclTensor *input, const aclTensor *output)
+{
+    return guarded([&]() -> int32_t {
+        Plan *p = lookup(handle);
+        if (!p) return E::ACL_ERROR_INVALID_PARAM;
+        std::lock_guard<std::mutex> lk(p->mu);
+        if (p->fftType != sip::kTypeC2C || p->istft.active) return E::ACL_ERROR_INVALID_PARAM;
+        if (!match(*p, input, ACL_COMPLEX64, false)) return E::ACL_ERROR_INVALID_PARAM;
+        if (!match(*p, output, ACL_COMPLEX64, false)) return E::ACL_ERROR_INVALID_PARAM;
+        if (!p->initialized) return E::ACL_ERROR_INVALID_PARAM;
+        if (p->ws_bytes > sip::kScratchBytes) {
+            SIP_LOG(ERROR, "sipFftExecC2CNoAlloc: the plan needs " << p->ws_bytes << " bytes of workspace, more than "
+                                                                   << "the " << sip::kScratchBytes
+                                                                   << "-byte scratch array");
+            return E::ACL_ERROR_INVALID_PARAM;
+        }
+        const void *src = aclGetTensorData(input);
+        void *dst = aclGetTensorData(output);
+        if (src == nullptr || dst == nullptr) return E::ACL_ERROR_INTERNAL_ERROR;
+        int cur = 0;
+        if (hipGetDevice(&cur) == hipSuccess && cur != p->device) SIP_HIP_CHECK(hipSetDevice(p->device));
+        void *const saved = p->workspace;
+        p->workspace = p->ws_bytes > 0 ? sip::scratch_device_ptr(p->device) : nullptr;
+        try {
+            sip::plan_execute(*p, src, dst);
+        } catch (...) {
+            p->workspace = saved;
+            throw;
+        }
+        p->workspace = saved;
+        return E::ACL_SUCCESS;
     });
 }
 
