@@ -109,7 +109,9 @@ int32_t sipFftSetPrecision(void *handle, int32_t mode);
  * (8-byte aligned), written to out_ptrs[i]. The buffers are bound at plan time.
  * flags: bit 0 = capture the whole batch into one hipGraph (each exec is one graph launch);
  *        bit 1 = autotune tile shapes per length group (decisions cached in $SIP_FFT_TUNE_CACHE,
- *                default $HOME/.sip_fft_tune.jsonl).
+ *                default $HOME/.sip_fft_tune.jsonl);
+ *        bit 2 = run every 2,3,5-smooth length up to 4096 in one persistent multi-length launch
+ *                (with bit 1 the radix order of each such length is tuned).
  * Exec only enqueues on the bound stream. Workspace: query after MakePlan, caller-allocated. */
 int32_t sipFftBatchCreate(void **handle);
 int32_t sipFftBatchSetStream(void *handle, void *stream);

@@ -61,6 +61,7 @@ PREC_FP32, PREC_F16_SPLIT, PREC_BF16_SPLIT, PREC_BF16_SPLIT3 = 0, 1, 2, 3
 
 BATCH_GRAPH = 1
 BATCH_AUTOTUNE = 2
+BATCH_PERSIST = 4
 
 _lib = None
 

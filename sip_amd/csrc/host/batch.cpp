@@ -8,10 +8,14 @@
 // each exec is one graph launch. An optional autotuner times the runtime tile-shape choices of every
 // group (transforms per workgroup, LDS layout, tile order, four-step split) on scratch rows of the
 // group's shape and keeps the fastest, caching decisions in a JSON file keyed by length, row count,
-// direction and device. This layer has no counterpart in the reference; it is an extension API (sipFftBatch*).
+// direction and device. With flag bit 2 every short length group (2,3,5-smooth, up to 4096) instead
+// joins one persistent multi-length launch (kernels/persist.hip) whose radix sequence per group is
+// runtime data, chosen by the tuner among candidate orders when autotuning. This layer has no
+// counterpart in the reference; it is an extension API (sipFftBatch*).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -20,6 +24,7 @@
 #include <sstream>
 #include <unordered_map>
 
+#include "persist.h"
 #include "plan.h"
 
 namespace sip {
@@ -40,6 +45,11 @@ struct BatchPlan {
     std::vector<int64_t> lengths, batches;
     std::vector<void *> ins, outs;
     std::vector<Group> groups;
+    // short length groups run by one persistent multi-length launch (flag bit 2), on chain
+    // persist_chain ahead of that chain's groups
+    std::unique_ptr<PersistLaunch> persist;
+    int persist_chain = 0;
+    std::vector<float> persist_tuned_us;
     int nchains = 1;
     std::vector<size_t> chain_ws;      // workspace bytes per chain
     std::vector<size_t> chain_ws_off;  // offset of each chain's region
@@ -110,7 +120,30 @@ std::string tune_key(int64_t n, int64_t rows, int direction)
 
 std::mutex g_cache_mu;
 std::map<std::string, Tune> g_tune_cache;
+std::map<std::string, std::vector<int>> g_radix_cache;  // persistent-launch radix sequences
 bool g_cache_loaded = false;
+
+// {"key":"...","radices":[a,b,...]}
+bool parse_radix_line(const std::string &line, std::string &key, std::vector<int> &rad)
+{
+    const std::string kk = "{\"key\":\"", rr = "\"radices\":[";
+    if (line.compare(0, kk.size(), kk) != 0) return false;
+    const size_t ke = line.find('"', kk.size());
+    const size_t rp = line.find(rr);
+    if (ke == std::string::npos || rp == std::string::npos) return false;
+    key = line.substr(kk.size(), ke - kk.size());
+    rad.clear();
+    size_t i = rp + rr.size();
+    while (i < line.size() && line[i] != ']') {
+        char *end = nullptr;
+        const long v = std::strtol(line.c_str() + i, &end, 10);
+        if (end == line.c_str() + i) return false;
+        rad.push_back((int)v);
+        i = (size_t)(end - line.c_str());
+        if (i < line.size() && line[i] == ',') i++;
+    }
+    return !rad.empty();
+}
 
 void load_cache()
 {
@@ -123,10 +156,15 @@ void load_cache()
         Tune t;
         long long l1 = 0;
         if (std::sscanf(line.c_str(),
-                        "{\"key\":\"%255[^\"]\",\"row_F\":%d,\"split_lds\":%d,\"xcd\":%d,\"split_l1\":%lld,\"col_F\":%d}",
-                        key, &t.row_F, &t.split_lds, &t.xcd, &l1, &t.col_F) == 6) {
+                        "{\"key\":\"%255[^\"]\",\"row_F\":%d,\"split_lds\":%d,\"xcd\":%d,\"split_l1\":%lld,\"col_F\":%d,"
+                        "\"rc\":%d}",
+                        key, &t.row_F, &t.split_lds, &t.xcd, &l1, &t.col_F, &t.rc) == 7) {
             t.split_l1 = l1;
             g_tune_cache[key] = t;
+        } else {
+            std::string k;
+            std::vector<int> rad;
+            if (parse_radix_line(line, k, rad)) g_radix_cache[k] = rad;
         }
     }
 }
@@ -137,7 +175,7 @@ void store_cache(const std::string &key, const Tune &t)
     std::ofstream f(cache_path(), std::ios::app);
     if (!f) return;  // an unwritable cache only costs re-tuning
     f << "{\"key\":\"" << key << "\",\"row_F\":" << t.row_F << ",\"split_lds\":" << t.split_lds << ",\"xcd\":" << t.xcd
-      << ",\"split_l1\":" << t.split_l1 << ",\"col_F\":" << t.col_F << "}\n";
+      << ",\"split_l1\":" << t.split_l1 << ",\"col_F\":" << t.col_F << ",\"rc\":" << t.rc << "}\n";
 }
 
 // ------------------------------------------------------------------ group plans
@@ -214,7 +252,8 @@ std::vector<Tune> candidates(int64_t n)
         }
     } else {
         // four-step splits n = a * b: the 8 most balanced ones (the default plan takes the most
-        // balanced pair of column leaves), each with three column-tile widths
+        // balanced pair of column leaves), each with three column-tile widths and the second stage on
+        // the rows-in / columns-out kernel or on the two-plane tile
         std::vector<std::pair<double, int64_t>> splits;
         for (int64_t a = 2; a <= 1024 && a < n; a++) {
             if (n % a || !find_instance(a)) continue;
@@ -226,12 +265,14 @@ std::vector<Tune> candidates(int64_t n)
         std::sort(splits.begin(), splits.end());
         if (splits.size() > 8) splits.resize(8);
         for (const auto &sp : splits)
-            for (int cf : {0, 16, 32}) {
-                Tune t;
-                t.split_l1 = sp.second;
-                t.col_F = cf;
-                c.push_back(t);
-            }
+            for (int cf : {0, 16, 32})
+                for (int rc : {0, 1}) {
+                    Tune t;
+                    t.split_l1 = sp.second;
+                    t.col_F = cf;
+                    t.rc = rc;
+                    c.push_back(t);
+                }
     }
     return c;
 }
@@ -294,6 +335,94 @@ void tune_group(BatchPlan &bp, Group &g, const TuneScratch &sc)
     store_cache(key, best);
 }
 
+// Radix order of one short group in the persistent launch: each candidate sequence
+// (persist_candidate_radices) runs the group alone on scratch rows, the fastest is kept and cached.
+std::vector<int> tune_persist(BatchPlan &bp, const Group &g, const TuneScratch &sc, float &best_us)
+{
+    const std::string key = "persist:" + tune_key(g.n, g.rows, bp.direction);
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        load_cache();
+        auto it = g_radix_cache.find(key);
+        if (it != g_radix_cache.end()) return it->second;
+    }
+    PersistSpec sp;
+    sp.n = g.n;
+    sp.in_rows.resize((size_t)g.rows);
+    for (int64_t r = 0; r < g.rows; r++) sp.in_rows[(size_t)r] = r * g.n;
+    sp.out_rows = sp.in_rows;
+    std::vector<int> best;
+    best_us = 1e30f;
+    hipEvent_t e0, e1;
+    SIP_HIP_CHECK(hipEventCreate(&e0));
+    SIP_HIP_CHECK(hipEventCreate(&e1));
+    try {
+        for (const std::vector<int> &cand : persist_candidate_radices(g.n, 6)) {
+            sp.radices = cand;
+            PersistLaunch pl({sp}, sc.in, sc.out, bp.direction == kDirInverse ? 1 : -1);
+            pl.launch(bp.stream);  // warm-up
+            SIP_HIP_CHECK(hipEventRecord(e0, bp.stream));
+            for (int r = 0; r < 5; r++) pl.launch(bp.stream);
+            SIP_HIP_CHECK(hipEventRecord(e1, bp.stream));
+            SIP_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0;
+            SIP_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms * 200.f < best_us) {
+                best_us = ms * 200.f;  // us per launch over 5 launches
+                best = cand;
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (best.empty()) return best;
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    g_radix_cache[key] = best;
+    std::ofstream f(cache_path(), std::ios::app);
+    if (f) {
+        f << "{\"key\":\"" << key << "\",\"radices\":[";
+        for (size_t i = 0; i < best.size(); i++) f << (i ? "," : "") << best[i];
+        f << "]}\n";
+    }
+    return best;
+}
+
+// The persistent launch over the short groups (largest first): rows at element offsets from the
+// lowest input / output pointer of those groups' problems.
+std::unique_ptr<PersistLaunch> build_persist(const BatchPlan &bp, const std::vector<Group> &pg,
+                                             const std::vector<std::vector<int>> &radices)
+{
+    uintptr_t in_lo = UINTPTR_MAX, out_lo = UINTPTR_MAX;
+    for (const Group &g : pg)
+        for (int i : g.members) {
+            in_lo = std::min(in_lo, reinterpret_cast<uintptr_t>(bp.ins[(size_t)i]));
+            out_lo = std::min(out_lo, reinterpret_cast<uintptr_t>(bp.outs[(size_t)i]));
+        }
+    std::vector<PersistSpec> specs(pg.size());
+    for (size_t k = 0; k < pg.size(); k++) {
+        const Group &g = pg[k];
+        PersistSpec &sp = specs[k];
+        sp.n = g.n;
+        sp.radices = radices[k];
+        sp.in_rows.reserve((size_t)g.rows);
+        sp.out_rows.reserve((size_t)g.rows);
+        for (int i : g.members) {
+            const int64_t ib = (int64_t)((reinterpret_cast<uintptr_t>(bp.ins[(size_t)i]) - in_lo) / 8);
+            const int64_t ob = (int64_t)((reinterpret_cast<uintptr_t>(bp.outs[(size_t)i]) - out_lo) / 8);
+            for (int64_t r = 0; r < bp.batches[(size_t)i]; r++) {
+                sp.in_rows.push_back(ib + r * g.n);
+                sp.out_rows.push_back(ob + r * g.n);
+            }
+        }
+    }
+    return std::make_unique<PersistLaunch>(specs, reinterpret_cast<const void *>(in_lo), reinterpret_cast<void *>(out_lo),
+                                           bp.direction == kDirInverse ? 1 : -1);
+}
+
 void capture(BatchPlan &bp)
 {
     // Chains run concurrently: chain 0 on the plan's stream, the others on side streams forked and
@@ -313,6 +442,7 @@ void capture(BatchPlan &bp)
             SIP_HIP_CHECK(hipEventRecord(bp.events[0], main));
             for (int c = 1; c < bp.nchains; c++) SIP_HIP_CHECK(hipStreamWaitEvent(bp.side[(size_t)c - 1], bp.events[0], 0));
         }
+        if (bp.persist) bp.persist->launch(bp.persist_chain == 0 ? main : bp.side[(size_t)bp.persist_chain - 1]);
         for (Group &g : bp.groups) {
             g.plan->stream = g.chain == 0 ? main : bp.side[(size_t)g.chain - 1];
             plan_execute(*g.plan, nullptr, nullptr);
@@ -421,19 +551,38 @@ int32_t sipFftBatchMakePlan(void *handle, int32_t count, const int64_t *lengths,
             g.rows += batches[i];
         }
         b->groups.clear();
+        b->persist.reset();
+        b->persist_tuned_us.clear();
+        std::vector<sip::Group> pgroups;  // short lengths for the persistent launch (flag bit 2)
         for (auto &kv : by) {
             kv.second.cost = 16.0 * (double)kv.second.n * (double)kv.second.rows;
-            b->groups.push_back(std::move(kv.second));
+            if ((flags & 4) && !sip::persist_default_radices(kv.second.n).empty())
+                pgroups.push_back(std::move(kv.second));
+            else
+                b->groups.push_back(std::move(kv.second));
         }
-        // chains: longest processing time first
-        b->nchains = (int)std::min<size_t>(4, b->groups.size());
         std::sort(b->groups.begin(), b->groups.end(), [](const sip::Group &x, const sip::Group &y) { return x.cost > y.cost; });
+        std::sort(pgroups.begin(), pgroups.end(), [](const sip::Group &x, const sip::Group &y) { return x.cost > y.cost; });
+        // chains: longest processing time first over the group launches and the persistent launch
+        // (one item whose cost is its groups' bytes)
+        double pcost = 0;
+        for (const sip::Group &g : pgroups) pcost += g.cost;
+        b->nchains = (int)std::min<size_t>(4, b->groups.size() + (pgroups.empty() ? 0 : 1));
         std::vector<double> load((size_t)b->nchains, 0.0);
-        for (sip::Group &g : b->groups) {
+        bool persist_placed = pgroups.empty();
+        auto place = [&](double cost) {
             const size_t c = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
-            g.chain = (int)c;
-            load[c] += g.cost + 2.0e5;  // + a fixed per-launch cost
+            load[c] += cost + 2.0e5;  // + a fixed per-launch cost
+            return (int)c;
+        };
+        for (sip::Group &g : b->groups) {
+            if (!persist_placed && pcost >= g.cost) {
+                b->persist_chain = place(pcost);
+                persist_placed = true;
+            }
+            g.chain = place(g.cost);
         }
+        if (!persist_placed) b->persist_chain = place(pcost);
         // plans (tuned or default); tuning needs a scratch workspace sized for the largest candidate
         // Tuning runs on scratch rows (input, output and workspace of the largest group), zeroed
         // once; the caller's buffers are not read or written before the first exec.
@@ -442,6 +591,7 @@ int32_t sipFftBatchMakePlan(void *handle, int32_t count, const int64_t *lengths,
         sip::TuneScratch sc;
         if (autotune) {
             size_t rows_bytes = 0, ws_bytes = 0;
+            for (const sip::Group &g : pgroups) rows_bytes = std::max(rows_bytes, (size_t)g.n * (size_t)g.rows * 8);
             for (sip::Group &g : b->groups) {
                 rows_bytes = std::max(rows_bytes, (size_t)g.n * (size_t)g.rows * 8);
                 // four-step and Bluestein intermediates of the default layout (host-only plan)
@@ -469,6 +619,15 @@ int32_t sipFftBatchMakePlan(void *handle, int32_t count, const int64_t *lengths,
                     sip::tune_group(*b, g, sc);
                 else
                     g.plan = sip::build_group_plan(*b, g, sip::Tune{});
+            }
+            if (!pgroups.empty()) {
+                std::vector<std::vector<int>> rad(pgroups.size());
+                for (size_t i = 0; i < pgroups.size(); i++) {
+                    float us = -1.f;
+                    if (autotune) rad[i] = sip::tune_persist(*b, pgroups[i], sc, us);
+                    b->persist_tuned_us.push_back(us);
+                }
+                b->persist = sip::build_persist(*b, pgroups, rad);
             }
         } catch (...) {
             if (scratch) {
@@ -519,7 +678,7 @@ int32_t sipFftBatchExec(void *handle)
         BatchPlan *b = sip::lookup_batch(handle);
         if (!b) return E::ACL_ERROR_INVALID_PARAM;
         std::lock_guard<std::mutex> lk(b->mu);
-        if (b->groups.empty()) return E::ACL_ERROR_INVALID_PARAM;
+        if (b->groups.empty() && !b->persist) return E::ACL_ERROR_INVALID_PARAM;
         if (b->ws_bytes > 0 && !b->workspace) return E::ACL_ERROR_INTERNAL_ERROR;
         if (b->side.size() + 1 < (size_t)b->nchains) {
             for (int c = (int)b->side.size() + 1; c < b->nchains; c++) {
@@ -567,13 +726,20 @@ int32_t sipFftBatchDescribe(void *handle, char *buf, size_t cap, size_t *needed)
         std::lock_guard<std::mutex> lk(b->mu);
         std::ostringstream os;
         os << "{\"problems\":" << b->lengths.size() << ",\"groups\":" << b->groups.size() << ",\"chains\":" << b->nchains
-           << ",\"graph\":" << (b->use_graph ? 1 : 0) << ",\"workspace\":" << b->ws_bytes << ",\"group_list\":[";
+           << ",\"graph\":" << (b->use_graph ? 1 : 0) << ",\"workspace\":" << b->ws_bytes;
+        if (b->persist) {
+            os << ",\"persist_chain\":" << b->persist_chain << ",\"persist_tuned_us\":[";
+            for (size_t i = 0; i < b->persist_tuned_us.size(); i++) os << (i ? "," : "") << b->persist_tuned_us[i];
+            os << "],\"persist\":" << b->persist->describe();
+        }
+        os << ",\"group_list\":[";
         for (size_t i = 0; i < b->groups.size(); i++) {
             const sip::Group &g = b->groups[i];
             os << (i ? "," : "") << "{\"n\":" << g.n << ",\"rows\":" << g.rows << ",\"chain\":" << g.chain
                << ",\"stages\":" << g.plan->stages.size() << ",\"row_F\":" << g.plan->tune.row_F
                << ",\"split_lds\":" << g.plan->tune.split_lds << ",\"xcd\":" << g.plan->tune.xcd
                << ",\"split_l1\":" << g.plan->tune.split_l1 << ",\"col_F\":" << g.plan->tune.col_F
+               << ",\"rc\":" << g.plan->tune.rc
                << ",\"tuned_us\":" << g.tuned_us << "}";
         }
         os << "]}";

@@ -50,20 +50,21 @@ static float2 root_f(int64_t m, int64_t n, int sign)
     return r;
 }
 
-// mfma: the instance runs its radix-16 passes on the matrix cores, whose lanes take every
-// W^{k r}, r = 0..15, of a pass from a [k][16] table.
-static TableRef pass_twiddles(const StockhamInstance *in, int sign, bool mfma = false)
+// Pass twiddles of a radix sequence of length L. Plain passes: k-major [k][W^1, W^2, W^4, ..] over
+// the powers of two below R, padded to an even count (the kernel multiplies the other r together);
+// each pass table stays 16-byte aligned. mfma: the radix-16 passes run on the matrix cores, whose
+// lanes take every W^{k r}, r = 0..15, of a pass from a [k][16] table.
+static TableRef twiddles_for(int64_t L, const std::vector<int> &radices, int sign, bool mfma)
 {
     TableRef t;
-    t.key = (mfma ? "twm:" : "twe:") + std::to_string(in->L);
-    for (int p = 0; p < in->npass; p++) t.key += "," + std::to_string(in->radix[p]);
+    t.key = (mfma ? "twm:" : "twe:") + std::to_string(L);
+    for (int r : radices) t.key += "," + std::to_string(r);
     t.key += ":" + std::to_string(sign);
-    t.gen = [in, sign, mfma]() {
+    t.gen = [L, radices, sign, mfma]() {
         std::vector<float2> v;
         int64_t ns = 1;
-        const int64_t L = in->L;
-        for (int p = 0; p < in->npass; p++) {
-            const int64_t R = in->radix[p];
+        for (int rr : radices) {
+            const int64_t R = rr;
             const int64_t step = L / (ns * R);
             if (mfma && R == 16) {
                 for (int64_t k = 0; k < ns; k++)
@@ -71,9 +72,6 @@ static TableRef pass_twiddles(const StockhamInstance *in, int sign, bool mfma = 
                 ns *= R;
                 continue;
             }
-            // k-major [k][W^1, W^2, W^4, ..] over the powers of two below R, padded to an even
-            // count (the kernel multiplies the other r together); each pass table stays 16-byte
-            // aligned.
             int nl = 0;
             while ((int64_t(1) << nl) < R) nl++;
             const int slots = (nl + 1) & ~1;
@@ -88,6 +86,23 @@ static TableRef pass_twiddles(const StockhamInstance *in, int sign, bool mfma = 
         return v;
     };
     return t;
+}
+
+static TableRef pass_twiddles(const StockhamInstance *in, int sign, bool mfma = false)
+{
+    return twiddles_for(in->L, std::vector<int>(in->radix, in->radix + in->npass), sign, mfma);
+}
+
+TableRef pass_twiddles_seq(int64_t L, const std::vector<int> &radices, int sign)
+{
+    return twiddles_for(L, radices, sign, false);
+}
+
+int64_t pass_twiddle_rows(int R)
+{
+    int nl = 0;
+    while ((1 << nl) < R) nl++;
+    return (nl + 1) & ~1;
 }
 
 static TableRef real_roots(int64_t L, int sign)
@@ -517,6 +532,17 @@ struct Builder {
         const bool real_rows = (io_mode == IO_R2C_EVEN && a.direct_in && a.tout.es == 1 && !a.tout.unit_f) ||
                                (io_mode == IO_C2R_EVEN && a.direct_out && a.tin.es == 1 && !a.tin.unit_f);
         if (real_rows && inst->real_rows[0] != nullptr && env_flag("SIP_FFT_SPLIT_REAL", true)) a.split_lds = 1;
+        // Rows in, columns out (the second four-step stage: contiguous rows of the intermediate,
+        // transposed store): one plane, the first pass straight from HBM along each row, the last
+        // pass straight to HBM across adjacent transforms (instead of the two-plane tile with
+        // LDS-staged store).
+        st.rc = io_mode == IO_C2C && !mf && !in_sep && !out_sep && a.tin.es == 1 && !a.tin.unit_f &&
+                a.tout.unit_f && tw4_ntot == 0 && inst->rc[0] != nullptr &&
+                (p.tune.rc >= 0 ? p.tune.rc == 1 : env_flag("SIP_FFT_RC", true));
+        if (st.rc) {
+            a.split_lds = 0;
+            a.col_direct = 0;
+        }
         a.xcd_remap = p.tune.xcd >= 0 ? p.tune.xcd == 1 : env_flag("SIP_FFT_XCD", false);
         st.tw = pass_twiddles(inst, sign, mf != nullptr);
         if (mf) st.mfw = mfma_dft_table(p.precision, sign);
@@ -531,8 +557,8 @@ struct Builder {
         a.col_fixed = ((F * T) % F) == 0 ? 1 : 0;  // always true: block = F * T
         st.grid = dim3((unsigned)((nfft + F - 1) / F));
         st.block = dim3((unsigned)(F * T));
-        st.lds = a.split_lds ? (size_t)F * lp * sizeof(float)
-                             : (size_t)2 * F * lp * sizeof(float) + (size_t)3 * F * sizeof(int64_t);
+        st.lds = (a.split_lds || st.rc) ? (size_t)F * lp * sizeof(float)
+                                        : (size_t)2 * F * lp * sizeof(float) + (size_t)3 * F * sizeof(int64_t);
         if (const char *e = std::getenv("SIP_FFT_LDS_MIN_KB")) {
             // occupancy throttle for tuning: allocate at least this much LDS per workgroup
             const size_t m = (size_t)std::atoi(e) * 1024;
@@ -982,6 +1008,7 @@ Plan::~Plan()
 static const void *stage_kernel(const Stage &s)
 {
     if (s.inst2d) return s.sign < 0 ? s.inst2d->fwd : s.inst2d->inv;
+    if (s.rc) return s.inst->rc[s.sign < 0 ? 0 : 1];
     if (s.sa.split_lds && s.sa.col_direct) return s.inst->cols[(s.sa.tw4_ntot != 0 ? 2 : 0) + (s.sign < 0 ? 0 : 1)];
     if (s.sa.split_lds && s.sa.io_mode == IO_R2C_EVEN) return s.inst->real_rows[s.sign < 0 ? 0 : 1];
     if (s.sa.split_lds && s.sa.io_mode == IO_C2R_EVEN) return s.inst->real_rows[s.sign < 0 ? 2 : 3];
@@ -1159,7 +1186,7 @@ std::string plan_describe(const Plan &p)
                << ",\"grid\":" << s.grid.x << ",\"block\":" << s.block.x << ",\"unit_f_in\":" << s.sa.tin.unit_f
                << ",\"unit_f_out\":" << s.sa.tout.unit_f << ",\"tw4\":" << s.sa.tw4_ntot
                << ",\"direct_in\":" << s.sa.direct_in << ",\"direct_out\":" << s.sa.direct_out
-               << ",\"split_lds\":" << s.sa.split_lds << ",\"col_direct\":" << s.sa.col_direct
+               << ",\"split_lds\":" << s.sa.split_lds << ",\"col_direct\":" << s.sa.col_direct << ",\"rc\":" << (s.rc ? 1 : 0)
                << ",\"in_sep\":" << s.sa.in_sep << ",\"out_sep\":" << s.sa.out_sep << ",\"mfma\":" << (s.mfw.used() ? 1 : 0)
                << ",\"xcd_remap\":" << s.sa.xcd_remap;
             if (s.inst2d) os << ",\"tile2d\":[" << s.inst2d->X << "," << s.inst2d->Y << "]";

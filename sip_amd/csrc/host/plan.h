@@ -65,6 +65,7 @@ struct Tune {
     int xcd = -1;          // XCD-contiguous tile order
     int64_t split_l1 = 0;  // forced first factor of a four-step split
     int col_F = 0;         // transforms per workgroup for column (strided) launches
+    int rc = -1;           // rows-in / columns-out kernel for the second four-step stage
 };
 
 enum class StageKind { Stockham, Aux };
@@ -82,6 +83,8 @@ struct Stage {
     // fused 2D tile stage (both axes of X x Y slices in one launch); tw2 = column twiddles
     const Stockham2dInstance *inst2d = nullptr;
     TableRef tw2;
+    // rows-in / columns-out kernel (four-step second stage: contiguous rows, transposed store)
+    bool rc = false;
     int in_tab = -1, out_tab = -1;  // row tables (Plan::row_tables) gathered by the in / out side
     // Aux
     AuxArgs xa{};
@@ -162,6 +165,10 @@ const StockhamInstance *find_mfma_instance(int64_t L, int mode);
 // Fused 2D tile instance for X x Y slices (X, Y in 32, 64, 128), or null.
 const Stockham2dInstance *find_2d_instance(int64_t X, int64_t Y);
 TableRef mfma_dft_table(int mode, int sign);
+// Pass twiddles of an arbitrary radix sequence of length L (the leaf kernels' layout; the table of
+// a leaf with the same sequence is the same table), and the rows per k of a radix-R pass.
+TableRef pass_twiddles_seq(int64_t L, const std::vector<int> &radices, int sign);
+int64_t pass_twiddle_rows(int R);
 int64_t instance_count();
 
 // Launchers implemented in the .hip files.

@@ -143,6 +143,10 @@ COLUMN_LEAF_MAX = 1024
 # half lengths (with 32 or more threads per transform) of the real transforms that also get
 # single-plane R2C / C2R row kernels
 REAL_ROWS_MAX = 8192
+# lengths up to COLUMN_LEAF_MAX with two or more passes and at least this many threads per
+# transform (64-byte runs per row read) also get the rows-in / columns-out kernel of the second
+# four-step stage
+RC_MIN_THREADS = 8
 
 
 def main():
@@ -195,7 +199,15 @@ def main():
                 real = "{" + ", ".join(f"(const void *)&{nm}" for nm in names) + "}"
             else:
                 real = "{nullptr, nullptr, nullptr, nullptr}"
-            tab.append(f"    {{{n}, {t}, {len(seq)}, {{{radix}}}, (const void *)&{kf}, (const void *)&{ki}, {rows}, {cols}, {real}}},")
+            if n <= COLUMN_LEAF_MAX and len(seq) >= 2 and t >= RC_MIN_THREADS:
+                names = [f"stockham_rc_kernel<{n}, {t}, {sg}, {rs}>" for sg in (-1, 1)]
+                for nm in names:
+                    lines.append(f"template __global__ void {nm}(const StockhamArgs);")
+                rc = "{" + ", ".join(f"(const void *)&{nm}" for nm in names) + "}"
+            else:
+                rc = "{nullptr, nullptr}"
+            tab.append(f"    {{{n}, {t}, {len(seq)}, {{{radix}}}, (const void *)&{kf}, (const void *)&{ki}, {rows}, {cols}, "
+                       f"{real}, {rc}}},")
         lines.append(f"extern const StockhamInstance g_stockham_part_{k}[] = {{")
         lines += tab
         lines.append("};")

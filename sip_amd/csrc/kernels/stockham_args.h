@@ -106,6 +106,8 @@ struct StockhamInstance {
                              // with the four-step twiddle (L <= 1024 only, else null)
     const void *real_rows[4];  // single-plane R2C fwd / inv, C2R fwd / inv rows (L <= 8192 with
                                // T >= 32 only, else null)
+    const void *rc[2];         // single-plane rows-in / columns-out fwd / inv (four-step second
+                               // stage; L <= 1024 with two or more passes and T >= 8, else null)
 };
 
 // Registry entry of a fused 2D tile instance (stockham2d.h). The radix sequences, and so the

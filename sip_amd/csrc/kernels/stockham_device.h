@@ -354,13 +354,29 @@ struct RegTail {
     float2 (*o)[R];
 };
 
+// Thread roles of a single-plane chain. NoRemap: every pass keeps the thread's (transform, thread
+// in transform). LastRemap: the exchange before the last pass hands the thread another transform
+// and thread index (plane offset lbase, thread t), so the first and the last pass may walk the
+// tile in different lane orders (rows in, columns out).
+struct NoRemap {
+    static constexpr bool kRemap = false;
+    int lbase, t;
+    bool active;
+};
+struct LastRemap {
+    static constexpr bool kRemap = true;
+    int lbase, t;
+    bool active;
+};
+
 // C2C transforms with one LDS plane per transform (half the LDS of the two-plane layout, so
 // twice as many workgroups fit on a CU): contiguous rows, or column tiles whose lanes walk adjacent
 // columns. The first pass reads HBM, the last writes HBM through io.store, and every exchange in
 // between moves the real parts through the plane, then the imaginary parts.
-template <int L, int T, int S, int NS, int R, int... Rest, class Io>
+template <int L, int T, int S, int NS, int R, int... Rest, class Io, class Rm = NoRemap>
 __device__ __forceinline__ void split_chain(float *pl, const float2 *__restrict__ tw, int lbase, int t, bool active,
-                                            float2 (&v)[(L / R + T - 1) / T][R], const Io &io)
+                                            float2 (&v)[(L / R + T - 1) / T][R], const Io &io,
+                                            const Rm &rm = Rm{0, 0, false})
 {
     constexpr int NBF = L / R;
     constexpr int NB = (NBF + T - 1) / T;
@@ -402,6 +418,10 @@ __device__ __forceinline__ void split_chain(float *pl, const float2 *__restrict_
         constexpr int NBF2 = L / R2;
         constexpr int NB2 = (NBF2 + T - 1) / T;
         constexpr bool EXACT2 = (NBF2 % T) == 0;
+        // roles in the next pass (a different transform / thread before the last pass of a remap)
+        constexpr bool kNow = Rm::kRemap && sizeof...(Rest) == 1;
+        const int lbase2 = kNow ? rm.lbase : lbase, t2 = kNow ? rm.t : t;
+        const bool active2 = kNow ? rm.active : active;
         float2 n[NB2][R2];
 #pragma unroll
         for (int plane = 0; plane < 2; plane++) {
@@ -420,11 +440,11 @@ __device__ __forceinline__ void split_chain(float *pl, const float2 *__restrict_
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < NB2; q++) {
-                const int b = t + q * T;
-                if (active && (EXACT2 || b < NBF2)) {
+                const int b = t2 + q * T;
+                if (active2 && (EXACT2 || b < NBF2)) {
 #pragma unroll
                     for (int r = 0; r < R2; r++) {
-                        const float x = pl[lbase + lds_idx_read<NBF2>(b, r)];
+                        const float x = pl[lbase2 + lds_idx_read<NBF2>(b, r)];
                         if (plane)
                             n[q][r].y = x;
                         else
@@ -433,7 +453,7 @@ __device__ __forceinline__ void split_chain(float *pl, const float2 *__restrict_
                 }
             }
         }
-        split_chain<L, T, S, NS * R, Rest...>(pl, tw + NS * tw_rows<R>(), lbase, t, active, n, io);
+        split_chain<L, T, S, NS * R, Rest...>(pl, tw + NS * tw_rows<R>(), lbase2, t2, active2, n, io, rm);
     }
 }
 
@@ -686,6 +706,51 @@ __global__ __launch_bounds__(1024) void stockham_cols_kernel(const StockhamArgs 
 {
     extern __shared__ float lds[];
     split_cols<L, T, S, TW4, Rs...>(a, lds, tile_index(a));
+}
+
+// Rows in, columns out (the second stage of a four-step split: transform k1 is a contiguous row of
+// the intermediate and its element k2 goes to k1 + L1 k2, so adjacent transforms are adjacent on
+// output). The first pass reads each row straight from HBM with the transform's T threads on
+// consecutive elements (thread tid: transform tid / T); the exchange before the last pass regroups
+// the threads so that lanes walk adjacent transforms (thread tid: transform tid mod F), and the last
+// pass writes straight to HBM, a whole run of F adjacent outputs per element. Needs two or more
+// passes (the regrouping happens in an exchange).
+template <int L, int T, int S, int R0, int... Rest>
+__device__ __forceinline__ void split_rows_cols(const StockhamArgs &a, float *pl, int64_t tile)
+{
+    static_assert(sizeof...(Rest) >= 1, "rows-in / columns-out needs two or more passes");
+    constexpr int NBF0 = L / R0;
+    constexpr int NB0 = (NBF0 + T - 1) / T;
+    constexpr bool EXACT0 = (NBF0 % T) == 0;
+    const int F = a.F;
+    const int tid = threadIdx.x;
+    const int fr = tid / T, tr = tid - fr * T;  // row roles
+    const int tc = (int)fdiv((uint32_t)tid, a.div_F);
+    const int fc = tid - tc * F;                // column roles
+    const int64_t f1 = tile * F + fr, f2 = tile * F + fc;
+    const bool act1 = f1 < a.nfft, act2 = f2 < a.nfft;
+    int64_t ib = 0, ob = 0;
+    if (act1) ib = tile_base((uint32_t)f1, a.tin);
+    if (act2) ob = tile_base((uint32_t)f2, a.tout);
+    const float2 *g = static_cast<const float2 *>(a.in) + ib;
+    float2 v[NB0][R0];
+#pragma unroll
+    for (int q = 0; q < NB0; q++) {
+        const int b = tr + q * T;
+        if (act1 && (EXACT0 || b < NBF0)) {
+#pragma unroll
+            for (int r = 0; r < R0; r++) v[q][r] = gload(g + b + r * NBF0);
+        }
+    }
+    const ColIoT<0> io{nullptr, static_cast<float2 *>(a.out) + ob, 0, a.tout.es, 0, &a};
+    split_chain<L, T, S, 1, R0, Rest...>(pl, a.tw, fr * a.LP, tr, act1, v, io, LastRemap{fc * a.LP, tc, act2});
+}
+
+template <int L, int T, int S, int... Rs>
+__global__ __launch_bounds__(1024) void stockham_rc_kernel(const StockhamArgs a)
+{
+    extern __shared__ float lds[];
+    split_rows_cols<L, T, S, Rs...>(a, lds, tile_index(a));
 }
 
 // Pass runners: the radix chain the tile goes through between load and store.

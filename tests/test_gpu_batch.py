@@ -168,7 +168,8 @@ def test_autotune_in_place(torch_cuda, tmp_path, monkeypatch):
 def test_batching_config_workload(torch_cuda, tmp_path, monkeypatch):
     """The batching configuration itself: 4096 C2C problems, lengths log-uniform over 2^a 3^b 5^c in
     [64, 65536], batch U[1, 64] (257 distinct lengths, 131,586 transforms), inputs uniform[-1, 1).
-    The autotuned hipGraph launch is bit-identical to the same tuned plans launched directly, and
+    The autotuned hipGraph launch (lengths up to 4096 in the persistent multi-length launch with
+    tuned radix orders) is bit-identical to the same tuned plans launched directly, and
     sampled rows of every length (the first and the last row of each length group, plus one random
     problem's rows) are within 2e-6 of the FP64 oracle for the graph and for the untuned grouped
     launches."""
@@ -193,7 +194,8 @@ def test_batching_config_workload(torch_cuda, tmp_path, monkeypatch):
 
     s = torch.cuda.current_stream().cuda_stream
     handles = []
-    for y, flags in ((y_graph, S.BATCH_GRAPH | S.BATCH_AUTOTUNE), (y_direct, S.BATCH_AUTOTUNE), (y_plain, 0)):
+    tuned = S.BATCH_AUTOTUNE | S.BATCH_PERSIST
+    for y, flags in ((y_graph, S.BATCH_GRAPH | tuned), (y_direct, tuned), (y_plain, 0)):
         bh = S.BatchHandle()
         assert bh.make_plan(lengths, batches, ins, outs(y), FWD, flags) == 0
         bh.alloc_workspace()
@@ -203,7 +205,8 @@ def test_batching_config_workload(torch_cuda, tmp_path, monkeypatch):
         handles.append(bh)
     torch.cuda.synchronize()
     d = handles[0].describe()
-    assert d["groups"] == len(set(lengths)) and d["graph"] == 1
+    assert d["groups"] + d["persist"]["groups"] == len(set(lengths)) and d["graph"] == 1
+    assert d["persist"]["groups"] == len({n for n in lengths if n <= 4096})
     assert torch.equal(y_graph, y_direct), "graph launch differs from the direct launches of the same plans"
     # sampled rows: first and last row of every length group, and all rows of one random problem
     first, last = {}, {}
@@ -228,3 +231,88 @@ def test_batching_config_workload(torch_cuda, tmp_path, monkeypatch):
     print(f"[parity] batching workload: {len(rows)} sampled rows, worst rel-L2 {worst:.3e}", flush=True)
     for bh in handles:
         bh.destroy()
+
+
+# short lengths for the persistent multi-length launch (2,3,5-smooth up to 4096), mixed with lengths
+# that stay group launches (1, a Bluestein length, a single-pass long leaf, a four-step length)
+P_LENGTHS = [8, 12, 20, 30, 64, 100, 128, 243, 625, 1000, 1536, 2000, 3000, 4096, 4096, 5000, 20000, 1, 17, 2, 3, 64]
+P_BATCHES = [5, 3, 7, 2, 9, 4, 1, 3, 2, 6, 2, 3, 1, 2, 5, 2, 1, 4, 3, 11, 6, 1]
+
+
+def _smooth(n):
+    for p in (2, 3, 5):
+        while n % p == 0:
+            n //= p
+    return n == 1
+
+
+P_SHORT = {n for n in P_LENGTHS if 2 <= n <= 4096 and _smooth(n)}
+
+
+@pytest.mark.parametrize("direction", [FWD, INV])
+def test_persistent_launch_matches_alone(torch_cuda, direction):
+    """The persistent launch with each length's default radix order runs the leaf kernels' passes
+    (same twiddle tables, same butterflies; the compiler may contract multiply-adds differently in
+    the two kernels): every problem is within 2e-6 of the FP64 oracle (5e-6 for the Bluestein length
+    17, which stays a group launch) and within 5e-7 (rel-L2) of the problem alone through
+    asdFftExecC2C. Graph and direct launches."""
+    torch = torch_cuda
+    short = P_SHORT
+    for flags in (S.BATCH_PERSIST, S.BATCH_PERSIST | S.BATCH_GRAPH):
+        hs, xs, ys = _problems(torch, P_LENGTHS, P_BATCHES, 700)
+        bh = _run_batch(torch, P_LENGTHS, P_BATCHES, xs, ys, direction, flags)
+        d = bh.describe()
+        assert d["persist"]["groups"] == len(short)
+        assert d["groups"] == len(set(P_LENGTHS)) - len(short)
+        for g in d["persist"]["group_list"]:
+            assert int(np.prod(g["radices"])) == g["n"] and g["F"] * g["T"] <= 256
+        for i, (n, b) in enumerate(zip(P_LENGTHS, P_BATCHES)):
+            got = ys[i].cpu().numpy()
+            rel, _ = report(f"persist flags={flags} problem {i} n={n} b={b}", got,
+                            oracle.c2c(hs[i], oracle.FORWARD if direction == FWD else oracle.INVERSE))
+            assert rel <= (5e-6 if n == 17 else TOL)
+            h = S.FftHandle()
+            assert h.make_plan_1d(n, S.ASCEND_FFT_C2C, direction, b) == 0
+            h.alloc_workspace()
+            h.use_current_torch_stream()
+            alone = torch.empty_like(ys[i])
+            assert h.exec_c2c(xs[i], alone) == 0
+            assert h.synchronize() == 0
+            rel_alone, _ = oracle.error(ys[i].cpu().numpy(), alone.cpu().numpy().astype(np.complex128))
+            assert rel_alone <= 5e-7, f"problem {i} (n={n}): persistent vs alone rel-L2 {rel_alone:.3e}"
+            h.destroy()
+        bh.destroy()
+
+
+def test_persistent_tuned_radix_orders(torch_cuda, tmp_path, monkeypatch):
+    """Autotuned radix orders (each candidate timed alone, the fastest kept and cached): results within
+    2e-6 of the oracle; the cache holds one radix line per short length; a second plan reuses it and
+    its graph replays (eager work and synchronisation in between) are bit-identical."""
+    torch = torch_cuda
+    cache = tmp_path / "tune.jsonl"
+    monkeypatch.setenv("SIP_FFT_TUNE_CACHE", str(cache))
+    flags = S.BATCH_PERSIST | S.BATCH_GRAPH | S.BATCH_AUTOTUNE
+    hs, xs, ys = _problems(torch, P_LENGTHS, P_BATCHES, 900)
+    bh = _run_batch(torch, P_LENGTHS, P_BATCHES, xs, ys, FWD, flags)
+    d = bh.describe()
+    short = P_SHORT
+    assert len(d["persist_tuned_us"]) == len(short) and all(u > 0 for u in d["persist_tuned_us"])
+    for i, (n, b) in enumerate(zip(P_LENGTHS, P_BATCHES)):
+        rel, _ = report(f"persist tuned n={n}", ys[i].cpu().numpy(), oracle.c2c(hs[i]))
+        assert rel <= (5e-6 if n == 17 else TOL)
+    lines = [ln for ln in cache.read_text().splitlines() if '"radices"' in ln]
+    assert len(lines) == len(short)
+    first = [y.clone() for y in ys]
+    bh.destroy()
+    bh = _run_batch(torch, P_LENGTHS, P_BATCHES, xs, ys, FWD, flags)    # decisions from the cache
+    assert [g["radices"] for g in bh.describe()["persist"]["group_list"]] == \
+        [g["radices"] for g in d["persist"]["group_list"]]
+    for _ in range(3):
+        for y in ys:
+            y.zero_()
+        torch.cuda.synchronize()
+        assert bh.exec() == 0
+        torch.cuda.synchronize()
+        for a, b in zip(first, ys):
+            assert torch.equal(a, b)
+    bh.destroy()
