@@ -544,8 +544,11 @@ struct Builder {
             a.col_direct = 0;
         }
         a.xcd_remap = p.tune.xcd >= 0 ? p.tune.xcd == 1 : env_flag("SIP_FFT_XCD", false);
+        // matrix passes read [k][16] twiddle rows; the single-plane rows kernel (K = 32) reads the
+        // interleaved operand table, the two-plane tile (K = 16) the blocked one
+        const bool mf2 = mf != nullptr && a.split_lds;
         st.tw = pass_twiddles(inst, sign, mf != nullptr);
-        if (mf) st.mfw = mfma_dft_table(p.precision, sign);
+        if (mf) st.mfw = mf2 ? mfma2_dft_table(p.precision, sign) : mfma_dft_table(p.precision, sign);
         if (io_mode == IO_R2C_EVEN || io_mode == IO_C2R_EVEN) st.rtw = real_roots(L, sign);
         if (tw4_ntot) {
             four_step_tables(tw4_ntot, sign, st.tw4lo, st.tw4hi);

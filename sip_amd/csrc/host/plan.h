@@ -165,6 +165,8 @@ const StockhamInstance *find_mfma_instance(int64_t L, int mode);
 // Fused 2D tile instance for X x Y slices (X, Y in 32, 64, 128), or null.
 const Stockham2dInstance *find_2d_instance(int64_t X, int64_t Y);
 TableRef mfma_dft_table(int mode, int sign);
+// Operand table of the single-plane K = 32 matrix passes (kernels/mfma_split.h).
+TableRef mfma2_dft_table(int mode, int sign);
 // Pass twiddles of an arbitrary radix sequence of length L (the leaf kernels' layout; the table of
 // a leaf with the same sequence is the same table), and the rows per k of a radix-R pass.
 TableRef pass_twiddles_seq(int64_t L, const std::vector<int> &radices, int sign);

@@ -103,4 +103,37 @@ TableRef mfma_dft_table(int mode, int sign)
     return t;
 }
 
+// Operands of the K = 32 radix-16 passes (kernels/mfma_split.h): [term][part][lane][8] 16-bit
+// values, part 0 = A_r (row q, column 2r: Re W[q][r], column 2r + 1: -Im W[q][r]), part 1 = A_i
+// (Im W[q][r], Re W[q][r]); lane l holds row l % 16, columns 8 (l / 16) .. + 7.
+TableRef mfma2_dft_table(int mode, int sign)
+{
+    TableRef t;
+    t.key = "mfw2:" + std::to_string(mode) + ":" + std::to_string(sign);
+    t.gen = [mode, sign]() {
+        const int nt = mode == PREC_BF16_SPLIT3 ? 3 : 2;
+        const bool f16 = mode == PREC_F16_SPLIT;
+        std::vector<uint16_t> v((size_t)nt * 2 * 64 * 8);
+        const long double two_pi = 6.283185307179586476925286766559005768L;
+        for (int part = 0; part < 2; part++)
+            for (int lane = 0; lane < 64; lane++)
+                for (int k8 = 0; k8 < 8; k8++) {
+                    const int q = lane % 16, col = 8 * (lane / 16) + k8, r = col / 2, c = col % 2;
+                    const long double a = two_pi * (long double)((q * r) % 16) / 16.0L;
+                    const double wr = (double)cosl(a), wi = (double)(sign * sinl(a));
+                    const double w = part == 0 ? (c == 0 ? wr : -wi) : (c == 0 ? wi : wr);
+                    double rest = w;
+                    for (int tm = 0; tm < nt; tm++) {
+                        const uint16_t b = f16 ? half_bits(rest) : bf16_bits(rest);
+                        v[(((size_t)tm * 2 + part) * 64 + lane) * 8 + k8] = b;
+                        rest -= f16 ? half_value(b) : bf16_value(b);
+                    }
+                }
+        std::vector<float2> out((v.size() + 3) / 4);
+        std::memcpy(out.data(), v.data(), v.size() * 2);
+        return out;
+    };
+    return t;
+}
+
 }  // namespace sip
