@@ -28,31 +28,44 @@ std::vector<std::vector<int>> persist_candidate_radices(int64_t n, size_t max_co
 // sequence does not fit the kernel (a radix it lacks, too many passes or values per thread).
 bool persist_shape(int64_t n, const std::vector<int> &radices, int &T, int &F);
 
+// Device image of one persistent launch: groups | tile -> group | input rows | output rows, with
+// the launch arguments and shape. Built on the host (persist_build_image), given its device address
+// (persist_place, which writes the image's own pointers), then copied there by the caller.
+struct PersistImage {
+    std::vector<char> bytes;
+    std::vector<PersistGroup> groups;        // host view (rows pointers are element indices)
+    std::vector<std::vector<int>> radices;
+    std::vector<TablePtr> tables;            // pass twiddles, held while the image is in use
+    size_t gbytes = 0, tbytes = 0, rbytes = 0;
+    PersistArgs args{};
+    size_t lds = 0;
+    unsigned grid = 1;
+    int sign = -1;
+};
+
+// in_base / out_base: device addresses the row offsets (complex elements) are relative to.
+void persist_build_image(const std::vector<PersistSpec> &specs, const void *in_base, void *out_base, int sign,
+                         PersistImage &img);
+void persist_place(PersistImage &img, void *dev_base);
+void persist_launch(const PersistImage &img, hipStream_t stream);
+
+// A persistent launch that owns its device image (batching form).
 class PersistLaunch {
 public:
-    // in_base / out_base: device addresses the row offsets (complex elements) are relative to.
     PersistLaunch(const std::vector<PersistSpec> &specs, const void *in_base, void *out_base, int sign);
     ~PersistLaunch();
     PersistLaunch(const PersistLaunch &) = delete;
     PersistLaunch &operator=(const PersistLaunch &) = delete;
 
     void launch(hipStream_t stream) const;
-    int64_t tiles() const { return args_.ntiles; }
-    size_t group_count() const { return host_.size(); }
-    unsigned grid() const { return grid_; }
-    size_t lds() const { return lds_; }
+    int64_t tiles() const { return img_.args.ntiles; }
+    size_t group_count() const { return img_.groups.size(); }
     // JSON fragment: the groups with their radices and shapes.
     std::string describe() const;
 
 private:
-    std::vector<PersistGroup> host_;
-    std::vector<std::vector<int>> radices_;
-    std::vector<TablePtr> tables_;
+    PersistImage img_;
     void *dev_ = nullptr;
-    PersistArgs args_{};
-    size_t lds_ = 0;
-    unsigned grid_ = 1;
-    int sign_ = -1;
 };
 
 }  // namespace sip

@@ -10,6 +10,10 @@
 //     single gathered-row plan (the row-offset tables are staged in pinned memory and copied
 //     asynchronously), so the kernels are the ones a request would use alone and each result is
 //     identical alone and in the mix;
+//   * optionally (SIP_FFT_SERVE_MIXED=1) coalesces queued C2C requests of different short lengths
+//     (2,3,5-smooth up to 4096) into one persistent multi-length launch (kernels/persist.hip; its
+//     descriptors staged the same way); a short C2C request alone then runs the same launch with
+//     one group, so here too each result is identical alone and in the mix;
 //   * spreads launches over several streams (default priority), so work admitted while other work
 //     runs overlaps on the device.
 // Plans are rebuilt per launch on the host (microseconds); the coefficient tables of every shape
@@ -19,6 +23,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -29,6 +34,7 @@
 #include <thread>
 #include <unordered_map>
 
+#include "persist.h"
 #include "plan.h"
 
 namespace sip {
@@ -113,6 +119,7 @@ struct Launch {
     size_t tab_off = SIZE_MAX, tab_bytes = 0;
     size_t pin_off = 0, pin_bytes = 0;
     std::unique_ptr<Plan> plan;
+    std::vector<TablePtr> hold;  // coefficient tables of a persistent launch
 };
 
 class Server {
@@ -237,13 +244,14 @@ public:
         std::ostringstream os;
         os << "{\"completed\":" << lat.size() << ",\"p50_ms\":" << pct(0.50) << ",\"p90_ms\":" << pct(0.90)
            << ",\"p99_ms\":" << pct(0.99) << ",\"max_ms\":" << (lat.empty() ? 0.0 : lat.back())
-           << ",\"launches\":" << launches_ << ",\"coalesced_requests\":" << coalesced_ << ",\"queued\":" << queue_.size()
+           << ",\"launches\":" << launches_ << ",\"mixed_launches\":" << mixed_launches_
+           << ",\"coalesced_requests\":" << coalesced_ << ",\"queued\":" << queue_.size()
            << ",\"inflight_launches\":" << inflight_.size() << ",\"arena_used\":" << arena_->used()
            << ",\"arena_peak\":" << arena_peak_ << ",\"budget\":" << budget_ << ",\"bytes_done\":" << bytes_done_
            << ",\"flops_done\":" << flops_done_ << "}";
         if (reset && os.str().size() + 1 <= cap) {
             lat_ms_.clear();
-            launches_ = coalesced_ = 0;
+            launches_ = coalesced_ = mixed_launches_ = 0;
             bytes_done_ = flops_done_ = 0;
             arena_peak_ = arena_->used();
         }
@@ -353,25 +361,27 @@ private:
                 }
             if (sidx < 0) break;
             Req *head = queue_.front();
-            // coalesce queued requests of the same shape (FIFO order, bounded scan)
+            // coalesce queued requests of the same shape (FIFO order, bounded scan), or, for short
+            // C2C lengths, of any short C2C length (one persistent multi-length launch)
             std::vector<Req *> group;
             int64_t rows = 0;
+            const bool mixed = mixed_ok(head);
             const bool gathered = gathered_supported(head->type, head->n);
             size_t scanned = 0;
             for (auto it = queue_.begin(); it != queue_.end() && scanned < 256; ++it, ++scanned) {
                 Req *r = *it;
-                if (r->type != head->type || r->n != head->n) continue;
-                if (!group.empty() && (!gathered || rows + r->batch > max_rows_)) break;
+                if (mixed ? !mixed_ok(r) : (r->type != head->type || r->n != head->n)) continue;
+                if (!group.empty() && ((!gathered && !mixed) || rows + r->batch > max_rows_)) break;
                 group.push_back(r);
                 rows += r->batch;
-                if (!gathered) break;
+                if (!gathered && !mixed) break;
             }
-            bool ok = launch_locked(group, rows, sidx);
+            bool ok = mixed ? launch_mixed_locked(group, sidx) : launch_locked(group, rows, sidx);
             if (!ok && group.size() > 1) {
                 // the coalesced launch does not fit now: try the head request alone
                 group.resize(1);
                 rows = head->batch;
-                ok = launch_locked(group, rows, sidx);
+                ok = mixed ? launch_mixed_locked(group, sidx) : launch_locked(group, rows, sidx);
             }
             if (!ok) {
                 if (!inflight_.empty()) break;  // out of memory: wait for completions to free some
@@ -528,6 +538,117 @@ private:
         return true;
     }
 
+    // Short C2C requests (2,3,5-smooth n up to 4096) run in persistent multi-length launches.
+    bool mixed_ok(const Req *r)
+    {
+        if (!mixed_enabled_ || r->type != kTypeC2C || r->n > kPersistMaxL) return false;
+        auto it = mixed_shape_.find(r->n);
+        if (it == mixed_shape_.end()) it = mixed_shape_.emplace(r->n, !persist_default_radices(r->n).empty()).first;
+        return it->second;
+    }
+
+    // One persistent launch over the group's requests (their lengths may differ): outputs the server
+    // owns from the arena, the launch image (descriptors, tile list, row tables) staged through the
+    // pinned ring and copied to the arena on the launch stream. False when memory is short now.
+    bool launch_mixed_locked(const std::vector<Req *> &group, int sidx)
+    {
+        auto l = std::make_unique<Launch>();
+        l->stream = sidx;
+        auto rollback = [&]() {
+            for (Req *r : group)
+                if (r->own_out && r->out_off != SIZE_MAX) {
+                    arena_->release(r->out_off, r->out_bytes);
+                    r->out_off = SIZE_MAX;
+                    r->out = nullptr;
+                }
+            if (l->tab_off != SIZE_MAX) arena_->release(l->tab_off, l->tab_bytes);
+            if (l->pin_bytes) pin_release(l->pin_off, l->pin_bytes);
+        };
+        for (Req *r : group) {
+            if (!r->own_out) continue;
+            r->out_off = arena_->alloc(r->out_bytes);
+            if (r->out_off == SIZE_MAX) {
+                rollback();
+                return false;
+            }
+            r->out = static_cast<char *>(arena_base_) + r->out_off;
+        }
+        try {
+            uintptr_t in_lo = UINTPTR_MAX, out_lo = UINTPTR_MAX;
+            for (Req *r : group) {
+                in_lo = std::min(in_lo, reinterpret_cast<uintptr_t>(r->in));
+                out_lo = std::min(out_lo, reinterpret_cast<uintptr_t>(r->out));
+            }
+            std::map<int64_t, PersistSpec> by;
+            for (Req *r : group) {
+                PersistSpec &sp = by[r->n];
+                sp.n = r->n;
+                const int64_t ib = (int64_t)((reinterpret_cast<uintptr_t>(r->in) - in_lo) / 8);
+                const int64_t ob = (int64_t)((reinterpret_cast<uintptr_t>(r->out) - out_lo) / 8);
+                for (int64_t k = 0; k < r->batch; k++) {
+                    sp.in_rows.push_back(ib + k * r->n);
+                    sp.out_rows.push_back(ob + k * r->n);
+                }
+            }
+            std::vector<PersistSpec> specs;
+            specs.reserve(by.size());
+            for (auto it = by.rbegin(); it != by.rend(); ++it) specs.push_back(std::move(it->second));  // longest first
+            PersistImage img;
+            persist_build_image(specs, reinterpret_cast<const void *>(in_lo), reinterpret_cast<void *>(out_lo), -1, img);
+            l->tab_bytes = granule(img.bytes.size());
+            l->tab_off = arena_->alloc(l->tab_bytes);
+            if (l->tab_off == SIZE_MAX || !pin_alloc(l->tab_bytes, &l->pin_off)) {
+                if (l->tab_off != SIZE_MAX) {
+                    arena_->release(l->tab_off, l->tab_bytes);
+                    l->tab_off = SIZE_MAX;
+                }
+                rollback();
+                return false;
+            }
+            l->pin_bytes = l->tab_bytes;
+            char *dimg = static_cast<char *>(arena_base_) + l->tab_off;
+            persist_place(img, dimg);
+            char *pin = static_cast<char *>(pin_) + l->pin_off;
+            std::memcpy(pin, img.bytes.data(), img.bytes.size());
+            hipStream_t st = streams_[(size_t)sidx];
+            SIP_HIP_CHECK(hipMemcpyAsync(dimg, pin, img.bytes.size(), hipMemcpyHostToDevice, st));
+            const double t = now_s();
+            for (Req *r : group) r->t_adm = t;
+            persist_launch(img, st);
+            l->hold = std::move(img.tables);
+            if (!ev_pool_.empty()) {
+                l->ev = ev_pool_.back();
+                ev_pool_.pop_back();
+            } else {
+                SIP_HIP_CHECK(hipEventCreateWithFlags(&l->ev, hipEventDisableTiming));
+            }
+            const hipError_t er = hipEventRecord(l->ev, st);
+            if (er != hipSuccess) {
+                ev_pool_.push_back(l->ev);
+                l->ev = nullptr;
+                SIP_HIP_CHECK(er);
+            }
+        } catch (...) {
+            (void)hipStreamSynchronize(streams_[(size_t)sidx]);
+            rollback();
+            for (Req *r : group) {
+                r->done = true;
+                r->status = AsdSip::ACL_ERROR_INTERNAL_ERROR;
+                r->t_done = now_s();
+            }
+            done_cv_.notify_all();
+            return true;
+        }
+        l->reqs = group;
+        inflight_per_stream_[(size_t)sidx]++;
+        launches_++;
+        mixed_launches_++;
+        coalesced_ += group.size();
+        inflight_.push_back(std::move(l));
+        arena_peak_ = std::max(arena_peak_, arena_->used());
+        return true;
+    }
+
     // The coefficient tables of every (type, n) served so far stay on the device for the server's
     // lifetime: a later launch of the same shape reuses them without a rebuild or an upload, and no
     // table is freed (a device-synchronising hipFree) while launches run.
@@ -543,6 +664,16 @@ private:
 
     size_t budget_;
     int nstreams_;
+    // Multi-length persistent launches only with SIP_FFT_SERVE_MIXED=1. Measured on MI355X (10 s
+    // Poisson windows of the serving configuration): they cut launches at 50,000 req/s from 481k
+    // to 341k but raised p99 (10k req/s 0.24 -> 0.88 ms, 50k 1.27 -> 2.66 ms), so single-length
+    // launches stay the default.
+    bool mixed_enabled_ = [] {
+        const char *e = std::getenv("SIP_FFT_SERVE_MIXED");
+        return e && e[0] == '1';
+    }();
+    std::map<int64_t, bool> mixed_shape_;  // n -> runs in persistent launches
+    uint64_t mixed_launches_ = 0;
     std::map<std::pair<int, int64_t>, std::vector<TablePtr>> resident_;
     int64_t max_rows_;
     int device_ = 0;

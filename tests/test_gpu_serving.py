@@ -1,7 +1,9 @@
 """Serving form (sipFftServe*): continuous batching of FFT requests.
 
-Each request's result is bit-identical whether it runs alone (plain plan) or coalesced with other
-requests in the server's mix (same kernels, same per-row arithmetic); a sample is also checked
+Each request's result is bit-identical whether it runs alone through the server or coalesced with
+other requests in the server's mix (same kernels, same per-row arithmetic: long lengths run the
+plain plan's kernels, short C2C lengths the persistent multi-length kernel whether alone or mixed),
+and within 5e-7 (rel-L2) of the same request through the plain API; a sample is also checked
 against the FP64 oracle (rel-L2 <= 2e-6). The open-loop benchmark completes every request it admits
 at a modest rate, and latency percentiles are ordered.
 """
@@ -29,6 +31,27 @@ def _alone(torch, t, n, b, x):
     assert h.synchronize() == 0
     h.destroy()
     return y
+
+
+def _served_alone(torch, srv, t, n, b, xd):
+    """The request alone through an idle server (nothing else queued or running)."""
+    assert srv.drain() == 0
+    if t == S.ASCEND_FFT_C2C:
+        y = torch.empty((b, n), dtype=torch.complex64, device="cuda")
+    else:
+        y = torch.empty((b, n // 2 + 1), dtype=torch.complex64, device="cuda")
+    st, _ = srv.wait(srv.submit(t, n, b, xd.data_ptr(), y.data_ptr()))
+    assert st == 0
+    return y
+
+
+def _check_alone(torch, srv, tag, t, n, b, xd, yd):
+    """Bit-identical to the request served alone; within 5e-7 of the plain API's result."""
+    alone = _served_alone(torch, srv, t, n, b, xd)
+    assert torch.equal(alone, yd), f"{tag} (type {t:#x}, n={n}, b={b}) differs alone vs in the mix"
+    plain = _alone(torch, t, n, b, xd)
+    rel, _ = oracle.error(yd.cpu().numpy(), plain.cpu().numpy().astype(np.complex128))
+    assert rel <= 5e-7, f"{tag}: served vs plain API rel-L2 {rel:.3e}"
 
 
 def test_alone_equals_mix(torch_cuda):
@@ -74,8 +97,7 @@ def test_alone_equals_mix(torch_cuda):
     stats = srv.stats()
     assert stats["coalesced_requests"] > stats["launches"], stats   # some launches carried several requests
     for i, (t, n, b, x, xd, yd) in enumerate(cases):
-        alone = _alone(torch, t, n, b, xd)
-        assert torch.equal(alone, yd), f"request {i} (type {t:#x}, n={n}, b={b}) differs alone vs in the mix"
+        _check_alone(torch, srv, f"request {i}", t, n, b, xd, yd)
         if i < 14:
             ref = oracle.c2c(x) if t == S.ASCEND_FFT_C2C else oracle.r2c(x)
             rel, _ = report(f"serving request {i} n={n}", yd.cpu().numpy(), ref)
@@ -189,8 +211,7 @@ def test_poisson_stream_2k_results(torch_cuda):
     assert stats["completed"] == len(reqs)
     for k, (x, xd, yd) in watched.items():
         _, ty, n, b = reqs[k]
-        alone = _alone(torch, ty, n, b, xd)
-        assert torch.equal(alone, yd), f"request {k} (type {ty:#x}, n={n}, b={b}) differs alone vs in the stream"
+        _check_alone(torch, srv, f"stream request {k}", ty, n, b, xd, yd)
         _check_rows(f"serving 2k request {k} n={n} b={b}", ty, n, x, yd)
     srv.destroy()
 
@@ -255,4 +276,36 @@ def test_coalesced_aux_lengths(torch_cuda):
         assert torch.equal(alone, yd), f"n={n}: coalesced result differs from the request alone"
         rel, _ = report(f"coalesced aux n={n}", yd.cpu().numpy(), oracle.c2c(x))
         assert rel <= (5e-6 if n in (17, 997) else 2e-6)
+    srv.destroy()
+
+
+def test_mixed_length_launches(torch_cuda, monkeypatch):
+    """With SIP_FFT_SERVE_MIXED=1, short C2C requests of different lengths (2,3,5-smooth up to 4096)
+    queued behind a long one coalesce into persistent multi-length launches: fewer launches than
+    requests, at least one mixed launch, and every result bit-identical to the request served
+    alone, within 5e-7 of the plain API and within 2e-6 of the FP64 oracle."""
+    torch = torch_cuda
+    monkeypatch.setenv("SIP_FFT_SERVE_MIXED", "1")
+    srv = S.Server(budget_bytes=4 << 30, nstreams=1, max_coalesce_rows=4096)
+    lengths = [256, 1536, 4096, 100, 3000, 243, 2000, 625, 4096, 256, 1000, 8, 720, 1024, 30, 64]
+    cases = []
+    for k, n in enumerate(lengths):
+        x, xd, yd = _watched_inputs(torch, S.ASCEND_FFT_C2C, n, 1 + (k * 7) % 13, 8500 + k)
+        cases.append((n, x, xd, yd))
+    torch.cuda.synchronize()
+    big = torch.zeros((64, 65536), dtype=torch.complex64, device="cuda")
+    tickets = [srv.submit(S.ASCEND_FFT_C2C, 65536, 64, big.data_ptr())]     # keeps the stream busy
+    for n, x, xd, yd in cases:
+        tickets.append(srv.submit(S.ASCEND_FFT_C2C, n, x.shape[0], xd.data_ptr(), yd.data_ptr()))
+    for tk in tickets:
+        st, _ = srv.wait(tk)
+        assert st == 0
+    stats = srv.stats()
+    print(f"mixed: {stats['launches']} launches ({stats['mixed_launches']} mixed) for {len(tickets)} requests",
+          flush=True)
+    assert stats["mixed_launches"] >= 1 and stats["launches"] < len(tickets), stats
+    for n, x, xd, yd in cases:
+        _check_alone(torch, srv, f"mixed n={n}", S.ASCEND_FFT_C2C, n, x.shape[0], xd, yd)
+        rel, _ = report(f"mixed n={n}", yd.cpu().numpy(), oracle.c2c(x))
+        assert rel <= 2e-6
     srv.destroy()
