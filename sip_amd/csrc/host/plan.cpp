@@ -463,13 +463,19 @@ struct Builder {
                 while (F > 1 && ((F + 1) / 2) * T >= 64 && (nfft + F - 1) / F < kFillWorkgroups) F = (F + 1) / 2;
             }
         }
-        if (mf) {
+        // column tiles of the precision form: single-plane tiles of 16, 32 or 64 adjacent
+        // transforms whose radix-16 passes take whole 16-transform blocks (K = 32 instructions)
+        const bool mf_cols = mf != nullptr && column && a.tin.unit_f && a.tout.unit_f && tw4_ntot == 0 &&
+                             !in_sep && !out_sep && inst->cols[0] != nullptr && F >= 16 && F % 16 == 0 &&
+                             F * T <= 1024 && (size_t)F * lp * sizeof(float) <= 160 * 1024 &&
+                             env_flag("SIP_FFT_MF_COLS", true);
+        if (mf && !mf_cols) {
             // every wave owns 4 blocks of 16 butterflies: F * T must be a multiple of 64
             while ((F * T) % 64 != 0) F++;
             while (F > 1 && ((size_t)F * per > 80 * 1024 || F * T > 1024) && ((F / 2) * T) % 64 == 0) F /= 2;
             if (F * T > 1024 || (size_t)F * per > 160 * 1024)
                 throw SipError(AsdSip::ACL_ERROR_API_NOT_SUPPORT, "no MFMA tile fits length " + std::to_string(L));
-        } else {
+        } else if (!mf_cols) {
             if (F > nfft) F = (int)nfft;
             if (F < 1) F = 1;
         }
@@ -486,9 +492,10 @@ struct Builder {
         if (io_mode == IO_C2R_FULL) a.tout.vec2 = 0;
         // First / last radix pass straight from / to HBM for contiguous complex rows, when a
         // transform has enough threads for its accesses to coalesce.
-        a.direct_in = (io_mode == IO_C2C || io_mode == IO_R2C_EVEN) && a.tin.es == 1 && !a.tin.unit_f && T >= 16;
+        a.direct_in = (io_mode == IO_C2C || io_mode == IO_R2C_EVEN) && a.tin.es == 1 && !a.tin.unit_f &&
+                      (T >= 16 || mf != nullptr);
         a.direct_out = (io_mode == IO_C2C || io_mode == IO_C2R_EVEN) && a.tout.es == 1 && !a.tout.unit_f &&
-                       T >= 16 && tw4_ntot == 0;
+                       (T >= 16 || mf != nullptr) && tw4_ntot == 0;
         // split-plane user buffers go through the LDS-staged load / store
         a.in_sep = in_sep ? 1 : 0;
         a.out_sep = out_sep ? 1 : 0;
@@ -503,7 +510,7 @@ struct Builder {
         const bool rows = io_mode == IO_C2C && a.direct_in && a.direct_out && inst->rows_fwd != nullptr;
         // column tiles (C2C, strided on both sides): first / last pass straight from / to HBM with
         // lanes walking adjacent columns
-        a.col_direct = (io_mode == IO_C2C && a.tin.unit_f && a.tout.unit_f && !mf && !in_sep && !out_sep &&
+        a.col_direct = (io_mode == IO_C2C && a.tin.unit_f && a.tout.unit_f && (!mf || mf_cols) && !in_sep && !out_sep &&
                         env_flag("SIP_FFT_COL_DIRECT", true)) ? 1 : 0;
         // Single-plane exchange (half the LDS, twice the resident workgroups) by default for rows
         // with 32 or more threads per transform. Measured on MI355X, 1 GiB of C2C rows: 4096
@@ -512,7 +519,7 @@ struct Builder {
         // XCD-contiguous tile order (4096 x 16384: 0.194 against 0.200 ms); the latter stays
         // available as the tuning switch SIP_FFT_XCD=1.
         a.split_lds = rows && (p.tune.split_lds >= 0 ? p.tune.split_lds == 1
-                                                     : env_flag("SIP_FFT_SPLIT_LDS", T >= 32));
+                                                     : env_flag("SIP_FFT_SPLIT_LDS", T >= 32 || mf != nullptr));
         // Column tiles with direct HBM passes: single plane too (a 1024 x 16 tile needs 70 KiB
         // instead of 139 KiB, so two workgroups fit on a CU). Measured on MI355X: 2D 1024^2 x 128
         // 892 -> 863 us, 1024^2 x 8 52 -> 48 us, 512^2 x 256 445 -> 422 us, 2^20 x 64 606 -> 566 us;

@@ -9,14 +9,17 @@ template <class... A>
 constexpr int count_args(A...) { return (int)sizeof...(A); }
 
 // fwd / inv: two-plane tiles (K = 16 instructions); rows_fwd / rows_inv: single-plane contiguous
-// rows with K = 32 instructions (lengths with 32 or more threads per transform)
+// rows with K = 32 instructions; cols[0..1]:
+// single-plane column tiles with K = 32 instructions (no four-step twiddle)
 #define SIP_MF(L, DT, NT, ...)                                                                     \
     {L, L / 16, count_args(__VA_ARGS__), {__VA_ARGS__},                                              \
      (const void *)&stockham_mfma_kernel<L, L / 16, -1, DT, NT, __VA_ARGS__>,                       \
      (const void *)&stockham_mfma_kernel<L, L / 16, 1, DT, NT, __VA_ARGS__>,                        \
-     (L / 16 >= 32) ? (const void *)&stockham_mf2_rows_kernel<L, L / 16, -1, DT, NT, __VA_ARGS__> : nullptr, \
-     (L / 16 >= 32) ? (const void *)&stockham_mf2_rows_kernel<L, L / 16, 1, DT, NT, __VA_ARGS__> : nullptr,  \
-     {nullptr, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}, {nullptr, nullptr}}
+     (const void *)&stockham_mf2_rows_kernel<L, L / 16, -1, DT, NT, __VA_ARGS__>,                   \
+     (const void *)&stockham_mf2_rows_kernel<L, L / 16, 1, DT, NT, __VA_ARGS__>,                    \
+     {(const void *)&stockham_mf2_cols_kernel<L, L / 16, -1, DT, NT, __VA_ARGS__>,                  \
+      (const void *)&stockham_mf2_cols_kernel<L, L / 16, 1, DT, NT, __VA_ARGS__>, nullptr, nullptr}, \
+     {nullptr, nullptr, nullptr, nullptr}, {nullptr, nullptr}}
 
 #define SIP_MF_SET(DT, NT)                                                                         \
     {SIP_MF(128, DT, NT, 16, 8), SIP_MF(256, DT, NT, 16, 16), SIP_MF(512, DT, NT, 16, 16, 2),       \

@@ -160,20 +160,34 @@ constexpr int mf2_tw_rows()
 }
 
 // Roles of a lane in the matrix layout: block u covers butterfly b[u] of tile transform fl[u].
-template <int T>
+// Rows (COLS = false): the 16 columns of a block are 16 consecutive butterflies, block u of wave w
+// holding tile butterflies (4 w + u) 16 + 0..15 (transform = index / T); when T >= 64 a wave lies in
+// one transform (FAST). Column tiles (COLS = true, F a multiple of 16): the 16 columns of a block
+// are 16 adjacent transforms at one butterfly, block index 4 w + u = b (F / 16) + transform group,
+// so that a wave's HBM accesses cover whole lines and its LDS accesses (transforms LP apart, LP
+// odd) fall in different banks.
+template <int T, bool COLS>
 struct MfLane {
+    static constexpr bool FAST = !COLS && T >= 64;
     int fl[4], b[4];
     int g, j;
-    __device__ __forceinline__ explicit MfLane(int tid)
+    __device__ __forceinline__ MfLane(int tid, int F)
     {
         const int wave = tid >> 6, lane = tid & 63;
         g = lane >> 4;
         j = lane & 15;
+        const int nfg = F >> 4;
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            const int gb = (wave * 4 + u) * 16 + j;
-            fl[u] = gb / T;
-            b[u] = gb - fl[u] * T;
+            if constexpr (COLS) {
+                const int beta = wave * 4 + u;
+                b[u] = beta / nfg;
+                fl[u] = (beta - b[u] * nfg) * 16 + j;
+            } else {
+                const int gb = (wave * 4 + u) * 16 + j;
+                fl[u] = gb / T;
+                b[u] = gb - fl[u] * T;
+            }
         }
     }
 };
@@ -184,11 +198,12 @@ struct MfLane {
 // butterflies t + q T). The exchanges move the outputs of one layout into the inputs of the next
 // through the plane (real parts, then imaginary parts). io: load / store of transform fl (tile
 // index) element j, and active(fl).
-template <int L, int T, int S, int DT, int NT, int NS, int R, int... Rest, class Io>
+template <int L, int T, int S, int DT, int NT, int NS, int R, int... Rest, class Io, class Ml>
 __device__ __forceinline__ void mf2_chain(float *pl, const float2 *__restrict__ tw, const MfW2<DT, NT> &W, int LP,
-                                          int lbase, int t, bool active, const MfLane<T> &ml,
+                                          int lbase, int t, bool active, const Ml &ml,
                                           float2 (&v)[(L / R + T - 1) / T][R], const Io &io)
 {
+    constexpr bool FAST = Ml::FAST;
     constexpr bool MF = R == 16;
     constexpr int NBF = L / R;
     constexpr int NB = (NBF + T - 1) / T;
@@ -234,7 +249,7 @@ __device__ __forceinline__ void mf2_chain(float *pl, const float2 *__restrict__ 
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int b = ml.b[u], k = b % NS, d = (b / NS) * NS * 16 + k;
-                if constexpr (T >= 64) {
+                if constexpr (FAST) {
                     // a wave lies inside one transform: the lane's own row
                     if (!active) continue;
 #pragma unroll
@@ -269,8 +284,8 @@ __device__ __forceinline__ void mf2_chain(float *pl, const float2 *__restrict__ 
             if constexpr (MF) {
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
-                    if (!(T >= 64 ? active : io.active(ml.fl[u]))) continue;
-                    const int b = ml.b[u], k = b % NS, hi = b / NS, lb = T >= 64 ? lbase : ml.fl[u] * LP;
+                    if (!(FAST ? active : io.active(ml.fl[u]))) continue;
+                    const int b = ml.b[u], k = b % NS, hi = b / NS, lb = FAST ? lbase : ml.fl[u] * LP;
 #pragma unroll
                     for (int tt = 0; tt < 4; tt++)
                         pl[lb + lds_idx_write<NS, 16>(hi, k, 4 * ml.g + tt)] =
@@ -294,7 +309,7 @@ __device__ __forceinline__ void mf2_chain(float *pl, const float2 *__restrict__ 
                 // reads its own plane slots, whatever they hold, and its results are never stored)
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
-                    const int lb = T >= 64 ? lbase : ml.fl[u] * LP;
+                    const int lb = FAST ? lbase : ml.fl[u] * LP;
 #pragma unroll
                     for (int tt = 0; tt < 4; tt++) {
                         const float x = pl[lb + lds_idx_read<NBF2>(ml.b[u], 4 * ml.g + tt)];
@@ -355,7 +370,7 @@ __device__ __forceinline__ void mf2_rows(const StockhamArgs &a, float *pl, int64
     const int tid = threadIdx.x;
     const int flp = tid / T;
     const int t = tid - flp * T;
-    const MfLane<T> ml(tid);
+    const MfLane<T, false> ml(tid, F);
     MfW2<DT, NT> W;
     W.load(a.mfma_w, tid & 63);
     Mf2RowIo io{static_cast<const float2 *>(a.in), static_cast<float2 *>(a.out), &a, tile, nullptr, nullptr};
@@ -366,7 +381,7 @@ __device__ __forceinline__ void mf2_rows(const StockhamArgs &a, float *pl, int64
     }
     float2 v[NB0][R0];
     if constexpr (R0 == 16) {
-        if constexpr (T >= 64) {
+        if constexpr (MfLane<T, false>::FAST) {
             // a wave lies inside one transform: the lane's own row (an inactive wave reads nothing)
             if (active) {
 #pragma unroll
@@ -409,6 +424,81 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SIP_MF2_WA
 {
     extern __shared__ float lds[];
     mf2_rows<L, T, S, DT, NT, Rs...>(a, lds, tile_index(a));
+}
+
+// Column tile (transforms adjacent, element j of transform fl at base(fl) + j es): element access
+// for the matrix layout (any tile transform) and the VALU layout (the thread's own transform).
+struct Mf2ColIo {
+    const float2 *in;
+    float2 *out;
+    const StockhamArgs *a;
+    int64_t tile;
+    const float2 *own_in;
+    float2 *own_out;
+    __device__ __forceinline__ bool active(int fl) const { return tile * a->F + fl < a->nfft; }
+    __device__ __forceinline__ const float2 *in_row(int fl) const
+    {
+        return in + tile_base((uint32_t)(tile * a->F + fl), a->tin);
+    }
+    __device__ __forceinline__ void store(int fl, int j, float2 v) const
+    {
+        gstore(out + tile_base((uint32_t)(tile * a->F + fl), a->tout) + (int64_t)j * a->tout.es, v);
+    }
+    __device__ __forceinline__ void store_own(int j, float2 v) const { gstore(own_out + (int64_t)j * a->tout.es, v); }
+};
+
+// Column tiles (no four-step twiddle) with the radix-16 passes on the matrix cores: the VALU passes
+// walk the tile with lanes on adjacent transforms (thread tid: transform tid mod F, thread tid / F),
+// the matrix passes with the column layout of MfLane; both first and last pass touch HBM directly.
+template <int L, int T, int S, int DT, int NT, int R0, int... Rest>
+__device__ __forceinline__ void mf2_cols(const StockhamArgs &a, float *pl, int64_t tile)
+{
+    constexpr int NBF0 = L / R0;
+    constexpr int NB0 = (NBF0 + T - 1) / T;
+    constexpr bool EXACT0 = (NBF0 % T) == 0;
+    const int F = a.F;
+    const int tid = threadIdx.x;
+    const int t = (int)fdiv((uint32_t)tid, a.div_F);
+    const int flp = tid - t * F;
+    const MfLane<T, true> ml(tid, F);
+    MfW2<DT, NT> W;
+    W.load(a.mfma_w, tid & 63);
+    Mf2ColIo io{static_cast<const float2 *>(a.in), static_cast<float2 *>(a.out), &a, tile, nullptr, nullptr};
+    const bool active = io.active(flp);
+    if (active) {
+        io.own_in = io.in_row(flp);
+        io.own_out = io.out + tile_base((uint32_t)(tile * F + flp), a.tout);
+    }
+    const int64_t esi = a.tin.es;
+    float2 v[NB0][R0];
+    if constexpr (R0 == 16) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            // a column beyond the launch's transforms re-reads the last transform (never stored)
+            const int fl = io.active(ml.fl[u]) ? ml.fl[u] : (int)(a.nfft - 1 - tile * F);
+            const float2 *g = io.in_row(fl) + (int64_t)ml.b[u] * esi;
+#pragma unroll
+            for (int tt = 0; tt < 4; tt++) v[0][4 * u + tt] = gload(g + (int64_t)((4 * ml.g + tt) * NBF0) * esi);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < NB0; q++) {
+            const int b = t + q * T;
+            if (active && (EXACT0 || b < NBF0)) {
+#pragma unroll
+                for (int r = 0; r < R0; r++) v[q][r] = gload(io.own_in + (int64_t)(b + r * NBF0) * esi);
+            }
+        }
+    }
+    mf2_chain<L, T, S, DT, NT, 1, R0, Rest...>(pl, a.tw, W, a.LP, flp * a.LP, t, active, ml, v, io);
+}
+
+template <int L, int T, int S, int DT, int NT, int... Rs>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SIP_MF2_WAVES))) void stockham_mf2_cols_kernel(
+    const StockhamArgs a)
+{
+    extern __shared__ float lds[];
+    mf2_cols<L, T, S, DT, NT, Rs...>(a, lds, tile_index(a));
 }
 
 }  // namespace sip
