@@ -129,7 +129,11 @@ int32_t sipFftBatchDestroy(void *handle);
  * requests of the same (type, n) are coalesced into one launch of at most max_coalesce_rows rows.
  * Submit: fftType C2C (0x10, complex64 in/out) or R2C (0x12, float in, n/2+1 bins out), forward;
  * input is a device address; output a device address, or NULL to let the server allocate it from the
- * arena (the result is then discarded after completion). Wait blocks until the request finished and
+ * arena (the result is then discarded after completion). SubmitMany hands over `count` requests
+ * at once (arrays of the Submit arguments; outputs may be NULL, or hold NULL entries): all are
+ * validated before any is queued (on a refusal nothing is queued and *refused, when given, is the
+ * index of the refused request), then they enter the queue together, so those of one shape can
+ * coalesce; tickets (may be NULL) receives one ticket per request. Wait blocks until the request finished and
  * returns its status and latency (arrival to observed completion). Stats returns JSON
  * (p50/p90/p99 latency, launches, coalescing, arena use); with reset = 1 the counters are cleared
  * only when the report fits `cap` (a size query with buf = NULL clears nothing). Benchmark runs an
@@ -139,6 +143,9 @@ int32_t sipFftBatchDestroy(void *handle);
 int32_t sipFftServeCreate(void **handle, size_t budget_bytes, int32_t nstreams, int64_t max_coalesce_rows);
 int32_t sipFftServeSubmit(void *handle, int32_t fftType, int64_t n, int64_t batch, const void *input, void *output,
                           uint64_t *ticket);
+int32_t sipFftServeSubmitMany(void *handle, int64_t count, const int32_t *fftTypes, const int64_t *n,
+                              const int64_t *batch, const void *const *inputs, void *const *outputs,
+                              uint64_t *tickets, int64_t *refused);
 int32_t sipFftServeWait(void *handle, uint64_t ticket, double *latency_ms);
 int32_t sipFftServeDrain(void *handle);
 int32_t sipFftServeStats(void *handle, int32_t reset, char *buf, size_t cap, size_t *needed);

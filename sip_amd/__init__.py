@@ -52,7 +52,8 @@ EXPORTED_SYMBOLS = [
     "aclFreeViewShape", "aclGetTensorData", "sipFftPlanDescribe", "sipFftPlanDescribeDry", "sipFftVersion",
     "sipFftBatchCreate", "sipFftBatchSetStream", "sipFftBatchMakePlan", "sipFftBatchGetWorkspaceSize",
     "sipFftBatchSetWorkspace", "sipFftBatchExec", "sipFftBatchDescribe", "sipFftBatchDestroy",
-    "sipFftServeCreate", "sipFftServeSubmit", "sipFftServeWait", "sipFftServeDrain", "sipFftServeStats",
+    "sipFftServeCreate", "sipFftServeSubmit", "sipFftServeSubmitMany", "sipFftServeWait", "sipFftServeDrain",
+    "sipFftServeStats",
     "sipFftServeBenchmark", "sipFftServeDestroy", "sipFftSetPrecision", "sipFftPlanTableDry",
     "sipFftCoeffCacheStats", "sipFftExecC2CNoAlloc",
 ]
@@ -112,6 +113,7 @@ def lib():
         "sipFftBatchDestroy": [vp],
         "sipFftServeCreate": [P(vp), sz, i32, i64],
         "sipFftServeSubmit": [vp, i32, i64, i64, vp, vp, P(ctypes.c_uint64)],
+        "sipFftServeSubmitMany": [vp, i64, P(i32), P(i64), P(i64), P(vp), P(vp), P(ctypes.c_uint64), P(i64)],
         "sipFftServeWait": [vp, ctypes.c_uint64, P(ctypes.c_double)],
         "sipFftServeDrain": [vp],
         "sipFftServeStats": [vp, i32, ctypes.c_char_p, sz, P(sz)],
@@ -488,6 +490,22 @@ class Server:
                                        ctypes.c_void_p(int(out_ptr) if out_ptr else None), ctypes.byref(t)),
                "sipFftServeSubmit")
         return t.value
+
+    def submit_many(self, requests):
+        """requests: (fft_type, n, batch, in_ptr, out_ptr or None) tuples, queued together (all or
+        none); returns their tickets."""
+        k = len(requests)
+        types = (ctypes.c_int32 * k)(*[int(r[0]) for r in requests])
+        ns = (ctypes.c_int64 * k)(*[int(r[1]) for r in requests])
+        bs = (ctypes.c_int64 * k)(*[int(r[2]) for r in requests])
+        ins = (ctypes.c_void_p * k)(*[int(r[3]) for r in requests])
+        outs = (ctypes.c_void_p * k)(*[int(r[4]) if r[4] else None for r in requests])
+        tks = (ctypes.c_uint64 * k)()
+        bad = ctypes.c_int64(-1)
+        st = lib().sipFftServeSubmitMany(self.h, k, types, ns, bs, ins, outs, tks, ctypes.byref(bad))
+        if st != ACL_SUCCESS:
+            raise SipFftError(f"sipFftServeSubmitMany (request {bad.value})", st)
+        return list(tks)
 
     def wait(self, ticket):
         lat = ctypes.c_double()

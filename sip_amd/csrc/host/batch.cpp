@@ -258,7 +258,7 @@ std::vector<Tune> candidates(int64_t n)
         for (int64_t a = 2; a <= 1024 && a < n; a++) {
             if (n % a || !find_instance(a)) continue;
             const int64_t b = n / a;
-            if (!find_instance(b) && b > 16384) continue;
+            if (b > 16384) continue;
             if (a * 16 < b || b * 16 < a) continue;  // keep splits within 16x of balanced
             splits.emplace_back(std::fabs(std::log((double)a) - std::log((double)b)), a);
         }

@@ -521,6 +521,7 @@ int32_t sipFftPlanDescribeDry(int32_t ndim, const int64_t *dims, int32_t fftType
             p.sizes.push_back(dims[i]);
         }
         sip::plan_build(p);
+        sip::plan_mark_streaming(p);
         return write_str(sip::plan_describe(p), buf, cap, needed);
     });
 }

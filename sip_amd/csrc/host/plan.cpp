@@ -353,6 +353,7 @@ bool env_flag(const char *name, bool dflt)
 
 constexpr int64_t kColumnLeafMax = 1024;
 constexpr int64_t kRowLeafMax = 16384;
+constexpr int64_t kBigLeafMinRows = 256;  // single-pass leaves above kRowLeafMax: transforms per launch
 constexpr int64_t kFillWorkgroups = 512;  // two workgroups per CU of MI355X (256 CUs)
 
 struct Builder {
@@ -384,9 +385,20 @@ struct Builder {
 
     bool leaf_ok(int64_t L, const Lay &in, const Lay &out) const
     {
-        if (!find_instance(L)) return false;
+        const StockhamInstance *inst = find_instance(L);
+        if (!inst) return false;
         if (is_column(in) || is_column(out)) return L <= kColumnLeafMax;
-        return L <= kRowLeafMax;
+        if (L <= kRowLeafMax) return true;
+        // Lengths above 16384 (2,3,5-smooth, up to 32768): one workgroup of up to 1024 threads per
+        // transform on the single-plane rows kernel (plane of 4 (L + L/16 + 1) bytes <= 136 KiB),
+        // for contiguous interleaved FP32 rows on both sides, from 256 transforms up (one
+        // workgroup per CU); everything else takes a four-step split. Measured on MI355X, one launch
+        // against the two four-step launches: 32768 x 4096 0.54 / 0.81 ms, 20000 x 6710 0.83 /
+        // 1.39 ms, 30375 x 4418 1.11 / 1.55 ms; with few transforms the split's parallelism wins
+        // (32768 x 16 23 / 14 us, 20000 x 64 25 / 20 us).
+        return inst->rows_fwd != nullptr && in.es == 1 && out.es == 1 && !is_user(in.buf) && !is_user(out.buf) &&
+               p.precision == PREC_FP32 && count(normalize(in.dims)) >= kBigLeafMinRows &&
+               env_flag("SIP_FFT_BIG_LEAF", true);
     }
 
     void add_stockham(int64_t L, int sign, const Lay &in, const Lay &out, int io_mode, int64_t in_len,
@@ -520,6 +532,12 @@ struct Builder {
         // available as the tuning switch SIP_FFT_XCD=1.
         a.split_lds = rows && (p.tune.split_lds >= 0 ? p.tune.split_lds == 1
                                                      : env_flag("SIP_FFT_SPLIT_LDS", T >= 32 || mf != nullptr));
+        if (L > kRowLeafMax) {
+            // single-pass leaves above 16384 exist only as the single-plane rows kernel
+            if (!rows)
+                throw SipError(AsdSip::ACL_ERROR_INTERNAL_ERROR, "length " + std::to_string(L) + " leaf off contiguous rows");
+            a.split_lds = 1;
+        }
         // Column tiles with direct HBM passes: single plane too (a 1024 x 16 tile needs 70 KiB
         // instead of 139 KiB, so two workgroups fit on a CU). Measured on MI355X: 2D 1024^2 x 128
         // 892 -> 863 us, 1024^2 x 8 52 -> 48 us, 512^2 x 256 445 -> 422 us, 2^20 x 64 606 -> 566 us;
@@ -674,7 +692,9 @@ struct Builder {
                 add_aux(AUX_CPLX_TRUNC, in, out, 1, 1, TableRef(), 1.f, "copy (length 1)");
             return;
         }
-        if (leaf_ok(L, in, out)) {
+        // a tuned split (split_l1) of a length above 16384 takes precedence over its single-pass leaf
+        const bool tuned_split = depth == 0 && L > kRowLeafMax && p.tune.split_l1 > 1;
+        if (!tuned_split && leaf_ok(L, in, out)) {
             add_stockham(L, sign, in, out, IO_C2C, L, L, 0, 1, "leaf " + std::to_string(L));
             return;
         }
@@ -1051,9 +1071,30 @@ void plan_materialize_external(Plan &p, const std::vector<const int64_t *> &dev_
     plan_materialize(p);
 }
 
+// Nontemporal final stores for a stage whose output no later stage of the plan reads (the single-plane
+// C2C rows kernel honours it). Measured on MI355X, 4096 x 16384 C2C: 188 against 193 us per launch;
+// the same stores on four-step intermediates and column tiles cost more than they gained (batching
+// workload 9.1 -> 10.2 ms), so only this last-pass row store takes them.
+void plan_mark_streaming(Plan &p)
+{
+    const bool on = env_flag("SIP_FFT_NT_OUT", true);
+    for (size_t i = 0; i < p.stages.size(); i++) {
+        Stage &s = p.stages[i];
+        if (s.kind != StageKind::Stockham) continue;
+        bool read_later = false;
+        for (size_t j = i + 1; j < p.stages.size(); j++) {
+            const BufRef &in = p.stages[j].in;
+            if (in.kind == s.out.kind && (in.kind == BUF_IN || in.kind == BUF_OUT || in.ws_index == s.out.ws_index))
+                read_later = true;
+        }
+        s.sa.nt_out = (on && !read_later) ? 1 : 0;
+    }
+}
+
 void plan_materialize(Plan &p)
 {
     SIP_HIP_CHECK(hipGetDevice(&p.device));
+    plan_mark_streaming(p);
     if (!p.external_tables.empty()) {
         for (Stage &s : p.stages) {
             const int64_t *ti = s.in_tab >= 0 ? p.external_tables[(size_t)s.in_tab] : nullptr;
@@ -1198,7 +1239,7 @@ std::string plan_describe(const Plan &p)
                << ",\"direct_in\":" << s.sa.direct_in << ",\"direct_out\":" << s.sa.direct_out
                << ",\"split_lds\":" << s.sa.split_lds << ",\"col_direct\":" << s.sa.col_direct << ",\"rc\":" << (s.rc ? 1 : 0)
                << ",\"in_sep\":" << s.sa.in_sep << ",\"out_sep\":" << s.sa.out_sep << ",\"mfma\":" << (s.mfw.used() ? 1 : 0)
-               << ",\"xcd_remap\":" << s.sa.xcd_remap;
+               << ",\"xcd_remap\":" << s.sa.xcd_remap << ",\"nt_out\":" << s.sa.nt_out;
             if (s.inst2d) os << ",\"tile2d\":[" << s.inst2d->X << "," << s.inst2d->Y << "]";
         } else {
             os << ",\"mode\":" << s.xa.mode << ",\"rows\":" << s.xa.rows << ",\"n\":" << s.xa.n

@@ -146,6 +146,9 @@ void plan_materialize_external(Plan &p, const std::vector<const int64_t *> &dev_
 void plan_build(Plan &p);
 // Creates the device-side coefficient tables and kernel attributes. Throws SipError.
 void plan_materialize(Plan &p);
+// Sets each stage's nontemporal-output flag (no later stage reads its output); plan_materialize
+// calls it, host-only descriptions call it directly.
+void plan_mark_streaming(Plan &p);
 // True when every stage's launch shape (threads, static + dynamic LDS) is within what the device
 // accepts for its kernel; checked before a tuning candidate is materialized or launched.
 bool plan_launchable(const Plan &p);

@@ -108,6 +108,7 @@ struct Req {
     size_t ws_need = 0;  // workspace of the request's plan when it runs alone
     double t_arr = 0, t_adm = 0, t_done = 0;
     bool done = false;
+    bool detached = false;  // nobody waits for it: released when it completes (load generation)
     int32_t status = 0;
 };
 
@@ -137,6 +138,7 @@ public:
             streams_.push_back(s);
         }
         inflight_per_stream_.assign((size_t)nstreams_, 0);
+        lat_ms_.reserve(size_t(1) << 22);  // a 60 s stream at 50,000 req/s without regrowing under the lock
         worker_ = std::thread([this] { loop(); });
     }
 
@@ -158,40 +160,44 @@ public:
         if (arena_base_) (void)hipFree(arena_base_);
     }
 
-    int32_t submit(int type, int64_t n, int64_t batch, const void *in, void *out, uint64_t *ticket)
+    // detached: the request is released when it completes (its latency still counts in the stats);
+    // nothing may wait for it. Load generators use this so that millions of finished requests do
+    // not accumulate in the ticket map.
+    int32_t submit(int type, int64_t n, int64_t batch, const void *in, void *out, uint64_t *ticket,
+                   bool detached = false)
     {
-        if ((type != kTypeC2C && type != kTypeR2C) || n <= 0 || n > kMaxFftSize || batch <= 0 || !in)
-            return AsdSip::ACL_ERROR_INVALID_PARAM;
-        auto r = std::make_unique<Req>();
-        r->type = type;
-        r->n = n;
-        r->batch = batch;
-        r->in = in;
-        r->out = out;
-        r->own_out = out == nullptr;
-        r->out_bytes = (size_t)batch * (size_t)(type == kTypeC2C ? n : n / 2 + 1) * 8;
-        // A request that could never be admitted alone is refused. Run alone, its launch needs its
-        // own output (when the server owns it) and the workspace of its plain plan; the layout is
-        // host-only work of a few microseconds.
-        try {
-            Plan dry;
-            dry.fftType = type;
-            dry.direction = kDirForward;
-            dry.sizes = {n};
-            dry.batch = batch;
-            plan_build(dry);
-            r->ws_need = dry.ws_bytes;
-        } catch (const SipError &e) {
-            return e.status;
-        }
-        const size_t need = (r->own_out ? granule(r->out_bytes) : 0) + (r->ws_need ? granule(r->ws_need) : 0);
-        if (need > budget_) return AsdSip::ACL_ERROR_INVALID_PARAM;
+        std::unique_ptr<Req> r;
+        const int32_t st = prepare(type, n, batch, in, out, r);
+        if (st != AsdSip::ACL_SUCCESS) return st;
+        r->detached = detached;
         r->t_arr = now_s();
         std::lock_guard<std::mutex> lk(mu_);
-        r->id = ++next_id_;
-        if (ticket) *ticket = r->id;
-        queue_.push_back(r.get());
-        reqs_[r->id] = std::move(r);
+        enqueue_locked(std::move(r), ticket);
+        cv_.notify_all();
+        return AsdSip::ACL_SUCCESS;
+    }
+
+    // Several requests handed over together: every one is validated first (nothing is queued when one
+    // is refused; `bad` receives its index), then all enter the queue under one lock, so the
+    // scheduler sees them at once and can coalesce those of one shape.
+    int32_t submit_many(int64_t count, const int32_t *types, const int64_t *ns, const int64_t *batches,
+                        const void *const *ins, void *const *outs, uint64_t *tickets, int64_t *bad)
+    {
+        if (count <= 0 || !types || !ns || !batches || !ins) return AsdSip::ACL_ERROR_INVALID_PARAM;
+        std::vector<std::unique_ptr<Req>> rs((size_t)count);
+        for (int64_t i = 0; i < count; i++) {
+            const int32_t st = prepare(types[i], ns[i], batches[i], ins[i], outs ? outs[i] : nullptr, rs[(size_t)i]);
+            if (st != AsdSip::ACL_SUCCESS) {
+                if (bad) *bad = i;
+                return st;
+            }
+        }
+        const double t = now_s();
+        std::lock_guard<std::mutex> lk(mu_);
+        for (int64_t i = 0; i < count; i++) {
+            rs[(size_t)i]->t_arr = t;
+            enqueue_locked(std::move(rs[(size_t)i]), tickets ? &tickets[i] : nullptr);
+        }
         cv_.notify_all();
         return AsdSip::ACL_SUCCESS;
     }
@@ -219,6 +225,7 @@ public:
             r->done = true;
             r->status = AsdSip::ACL_ERROR_API_NOT_SUPPORT;
             r->t_done = t;
+            if (r->detached) reqs_.erase(r->id);
         }
         queue_.clear();
         done_cv_.notify_all();
@@ -226,6 +233,20 @@ public:
     }
 
     double last_done() const { return last_done_; }
+
+    // Nothing queued and nothing running.
+    bool idle()
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        return queue_.empty() && inflight_.empty();
+    }
+
+    // Requests completed since creation: successful, failed.
+    std::pair<uint64_t, uint64_t> completed()
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        return {completed_ok_, completed_err_};
+    }
 
     void drain()
     {
@@ -239,11 +260,15 @@ public:
     {
         std::lock_guard<std::mutex> lk(mu_);
         std::vector<double> lat = lat_ms_;
-        std::sort(lat.begin(), lat.end());
-        auto pct = [&](double q) { return lat.empty() ? 0.0 : lat[(size_t)std::min<double>(lat.size() - 1, q * lat.size())]; };
+        auto pct = [&](double q) {
+            if (lat.empty()) return 0.0;
+            const size_t k = (size_t)std::min<double>(lat.size() - 1, q * lat.size());
+            std::nth_element(lat.begin(), lat.begin() + (std::ptrdiff_t)k, lat.end());
+            return lat[k];
+        };
         std::ostringstream os;
         os << "{\"completed\":" << lat.size() << ",\"p50_ms\":" << pct(0.50) << ",\"p90_ms\":" << pct(0.90)
-           << ",\"p99_ms\":" << pct(0.99) << ",\"max_ms\":" << (lat.empty() ? 0.0 : lat.back())
+           << ",\"p99_ms\":" << pct(0.99) << ",\"max_ms\":" << (lat.empty() ? 0.0 : *std::max_element(lat.begin(), lat.end()))
            << ",\"launches\":" << launches_ << ",\"mixed_launches\":" << mixed_launches_
            << ",\"coalesced_requests\":" << coalesced_ << ",\"queued\":" << queue_.size()
            << ",\"inflight_launches\":" << inflight_.size() << ",\"arena_used\":" << arena_->used()
@@ -261,6 +286,46 @@ public:
     int device() const { return device_; }
 
 private:
+    // A validated request, not yet queued. A request that could never be admitted alone is refused:
+    // run alone, its launch needs its own output (when the server owns it) and the workspace of its
+    // plain plan; the layout is host-only work of a few microseconds.
+    int32_t prepare(int type, int64_t n, int64_t batch, const void *in, void *out, std::unique_ptr<Req> &out_req)
+    {
+        if ((type != kTypeC2C && type != kTypeR2C) || n <= 0 || n > kMaxFftSize || batch <= 0 || !in)
+            return AsdSip::ACL_ERROR_INVALID_PARAM;
+        auto r = std::make_unique<Req>();
+        r->type = type;
+        r->n = n;
+        r->batch = batch;
+        r->in = in;
+        r->out = out;
+        r->own_out = out == nullptr;
+        r->out_bytes = (size_t)batch * (size_t)(type == kTypeC2C ? n : n / 2 + 1) * 8;
+        try {
+            Plan dry;
+            dry.fftType = type;
+            dry.direction = kDirForward;
+            dry.sizes = {n};
+            dry.batch = batch;
+            plan_build(dry);
+            r->ws_need = dry.ws_bytes;
+        } catch (const SipError &e) {
+            return e.status;
+        }
+        const size_t need = (r->own_out ? granule(r->out_bytes) : 0) + (r->ws_need ? granule(r->ws_need) : 0);
+        if (need > budget_) return AsdSip::ACL_ERROR_INVALID_PARAM;
+        out_req = std::move(r);
+        return AsdSip::ACL_SUCCESS;
+    }
+
+    void enqueue_locked(std::unique_ptr<Req> r, uint64_t *ticket)
+    {
+        r->id = ++next_id_;
+        if (ticket) *ticket = r->id;
+        queue_.push_back(r.get());
+        reqs_[r->id] = std::move(r);
+    }
+
     void loop()
     {
         (void)hipSetDevice(device_);
@@ -309,6 +374,11 @@ private:
                 bytes_done_ += (double)r->batch * (r->type == kTypeC2C ? 16.0 * r->n : 4.0 * r->n + 8.0 * (r->n / 2 + 1));
                 flops_done_ += (double)r->batch * (r->type == kTypeC2C ? 5.0 : 2.5) * r->n * std::log2((double)r->n);
                 if (r->own_out && r->out_off != SIZE_MAX) arena_->release(r->out_off, r->out_bytes);
+                if (r->status == AsdSip::ACL_SUCCESS)
+                    completed_ok_++;
+                else
+                    completed_err_++;
+                if (r->detached) reqs_.erase(r->id);
             }
             if (l.ws_off != SIZE_MAX) arena_->release(l.ws_off, l.ws_bytes);
             if (l.tab_off != SIZE_MAX) arena_->release(l.tab_off, l.tab_bytes);
@@ -486,7 +556,7 @@ private:
                 plan_build(*plan);
                 plan_materialize(*plan);
             }
-            keep_resident(type, n, *plan);
+            keep_resident(*plan);
             if (plan->ws_bytes) {
                 l->ws_bytes = plan->ws_bytes;
                 l->ws_off = arena_->alloc(l->ws_bytes);
@@ -649,17 +719,16 @@ private:
         return true;
     }
 
-    // The coefficient tables of every (type, n) served so far stay on the device for the server's
-    // lifetime: a later launch of the same shape reuses them without a rebuild or an upload, and no
-    // table is freed (a device-synchronising hipFree) while launches run.
-    void keep_resident(int type, int64_t n, const Plan &plan)
+    // Every coefficient table a launch has used stays on the device for the server's lifetime: a
+    // later launch reuses it without a rebuild or an upload, and no table is freed (a
+    // device-synchronising hipFree) while launches run. Tables are held one by one, not per shape:
+    // launches of one (type, n) can differ in plan (a single-pass leaf from 256 rows up, a four-step
+    // split below), and each variant's tables must stay.
+    void keep_resident(const Plan &plan)
     {
-        auto &held = resident_[std::make_pair(type, n)];
-        if (!held.empty()) return;
         for (const Stage &st : plan.stages)
             for (const TableRef *t : {&st.tw, &st.rtw, &st.tw4lo, &st.tw4hi, &st.mfw, &st.tab0, &st.tw2})
-                if (t->dev) held.push_back(t->dev);
-        if (held.empty()) held.push_back(nullptr);  // marks the shape as seen
+                if (t->dev) resident_.emplace(t->dev.get(), t->dev);
     }
 
     size_t budget_;
@@ -674,7 +743,7 @@ private:
     }();
     std::map<int64_t, bool> mixed_shape_;  // n -> runs in persistent launches
     uint64_t mixed_launches_ = 0;
-    std::map<std::pair<int, int64_t>, std::vector<TablePtr>> resident_;
+    std::unordered_map<const void *, TablePtr> resident_;
     int64_t max_rows_;
     int device_ = 0;
     void *arena_base_ = nullptr;
@@ -689,6 +758,7 @@ private:
     std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs_;
     uint64_t next_id_ = 0;
     std::vector<double> lat_ms_;
+    uint64_t completed_ok_ = 0, completed_err_ = 0;
     uint64_t launches_ = 0, coalesced_ = 0;
     double bytes_done_ = 0, flops_done_ = 0;
     size_t arena_peak_ = 0;
@@ -765,6 +835,17 @@ int32_t sipFftServeSubmit(void *handle, int32_t fftType, int64_t n, int64_t batc
         sip::Server *s = sip::lookup_server(handle);
         if (!s) return E::ACL_ERROR_INVALID_PARAM;
         return s->submit(fftType, n, batch, input, output, ticket);
+    });
+}
+
+int32_t sipFftServeSubmitMany(void *handle, int64_t count, const int32_t *fftTypes, const int64_t *n,
+                              const int64_t *batch, const void *const *inputs, void *const *outputs,
+                              uint64_t *tickets, int64_t *refused)
+{
+    return sip::guarded([&]() -> int32_t {
+        sip::Server *s = sip::lookup_server(handle);
+        if (!s) return E::ACL_ERROR_INVALID_PARAM;
+        return s->submit_many(count, fftTypes, n, batch, inputs, outputs, tickets, refused);
     });
 }
 
@@ -850,40 +931,28 @@ int32_t sipFftServeBenchmark(void *handle, double rate, double seconds, uint64_t
             gen.push_back(g);
         }
         (void)s->stats(true);
+        s->drain();
+        const auto c0 = s->completed();
         const double t0 = sip::now_s();
-        std::vector<uint64_t> tickets;
-        tickets.reserve(gen.size());
+        // requests are detached: released as they complete (latencies still enter the stats)
         for (const Gen &g : gen) {
             const double due = t0 + g.t;
             double now = sip::now_s();
             if (due > now) std::this_thread::sleep_for(std::chrono::duration<double>(due - now));
-            uint64_t tk = 0;
-            const int32_t st = s->submit(g.type, g.n, g.b, static_cast<const char *>(pool) + g.off, nullptr, &tk);
+            const int32_t st =
+                s->submit(g.type, g.n, g.b, static_cast<const char *>(pool) + g.off, nullptr, nullptr, true);
             if (st != E::ACL_SUCCESS) return st;
-            tickets.push_back(tk);
         }
         const double t_sub = sip::now_s();
         // Serve until the queue drains or the drain window closes; what is still queued then is
         // dropped (an overloaded stream would otherwise grow without bound), in-flight work finishes.
         const double limit = t0 + seconds + drain_seconds;
-        while (sip::now_s() < limit) {
-            std::string js = s->stats(false);
-            if (js.find("\"queued\":0,") != std::string::npos && js.find("\"inflight_launches\":0,") != std::string::npos)
-                break;
-            std::this_thread::sleep_for(std::chrono::milliseconds(2));
-        }
+        while (sip::now_s() < limit && !s->idle()) std::this_thread::sleep_for(std::chrono::milliseconds(1));
         const size_t dropped = s->cancel_queued();
         s->drain();
-        int32_t worst = E::ACL_SUCCESS;
-        size_t ok = 0;
-        for (uint64_t tk : tickets) {
-            double lat = 0;
-            const int32_t st = s->wait(tk, &lat);
-            if (st == E::ACL_SUCCESS)
-                ok++;
-            else if (st != E::ACL_ERROR_API_NOT_SUPPORT)
-                worst = st;
-        }
+        const auto c1 = s->completed();
+        const size_t ok = (size_t)(c1.first - c0.first);
+        const int32_t worst = c1.second > c0.second ? E::ACL_ERROR_INTERNAL_ERROR : E::ACL_SUCCESS;
         const double t_last = s->last_done();
         std::string st = s->stats(false);
         std::ostringstream os;

@@ -125,6 +125,59 @@ def elems_per_thread(n, seq, t):
     return max(((n // r + t - 1) // t) * r for r in seq)
 
 
+# 2,3,5-smooth lengths above MAX_SMOOTH up to BIG_ROWS_MAX get only the single-plane contiguous-row
+# kernel (one transform per workgroup of T <= 1024 threads; its plane, 4 (L + L/16 + 1) bytes, fits
+# the 160 KiB LDS up to 32768): a one-pass leaf where the two-plane tile would not fit. Their radix
+# sequences hold at most BIG_MAX_ELEMS values per thread (register budget of four waves per SIMD:
+# 77-112 VGPRs, no scratch, with radix-15 and -20 passes as well); lengths with no such sequence
+# (31104, 31250, 32000, 32400) stay four-step splits.
+BIG_ROWS_MAX = 32768
+BIG_RADICES = [20, 16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2]
+BIG_MAX_ELEMS = 32
+
+
+def big_lengths():
+    out = []
+    for a in range(16):
+        for b in range(11):
+            for c in range(8):
+                n = 2 ** a * 3 ** b * 5 ** c
+                if MAX_SMOOTH < n <= BIG_ROWS_MAX:
+                    out.append(n)
+    return sorted(out)
+
+
+def big_threads(n, seq):
+    """One butterfly of the largest radix per thread, at most 1024 threads."""
+    return min(1024, n // max(seq))
+
+
+def big_radix_sequence(n):
+    """Fewest passes over BIG_RADICES within BIG_MAX_ELEMS values per thread, then the fewest idle
+    butterfly rounds; largest radices first."""
+    best, best_key = None, None
+
+    def rec(m, maxr, seq):
+        nonlocal best, best_key
+        if m == 1:
+            t = big_threads(n, seq)
+            if elems_per_thread(n, seq, t) > BIG_MAX_ELEMS:
+                return
+            waste = sum((-(-(n // r) // t) * t - n // r) / (n // r) for r in seq)
+            key = (len(seq), waste)
+            if best_key is None or key < best_key:
+                best, best_key = list(seq), key
+            return
+        if len(seq) >= 8:
+            return
+        for r in BIG_RADICES:
+            if r <= maxr and m % r == 0:
+                rec(m // r, r, seq + [r])
+
+    rec(n, 1 << 30, [])
+    return best  # None: no sequence fits; the length stays a four-step split
+
+
 def instances():
     res = []
     for n in smooth_lengths():
@@ -133,6 +186,13 @@ def instances():
         e = elems_per_thread(n, seq, t)
         assert e <= 40, (n, seq, t, e)
         assert all(r in RADICES for r in seq), (n, seq)
+        res.append((n, t, seq))
+    for n in big_lengths():
+        seq = big_radix_sequence(n)
+        if seq is None:
+            continue
+        t = big_threads(n, seq)
+        assert t >= 32 and elems_per_thread(n, seq, t) <= BIG_MAX_ELEMS, (n, seq, t)
         res.append((n, t, seq))
     return res
 
@@ -171,11 +231,20 @@ def main():
         tab = []
         for n, t, seq in items:
             rs = ", ".join(str(r) for r in seq)
+            radix = ", ".join(str(r) for r in seq)
+            if n > MAX_SMOOTH:
+                rf = f"stockham_rows_kernel<{n}, {t}, -1, {rs}>"
+                ri = f"stockham_rows_kernel<{n}, {t}, 1, {rs}>"
+                lines.append(f"template __global__ void {rf}(const StockhamArgs);")
+                lines.append(f"template __global__ void {ri}(const StockhamArgs);")
+                tab.append(f"    {{{n}, {t}, {len(seq)}, {{{radix}}}, nullptr, nullptr, (const void *)&{rf}, "
+                           f"(const void *)&{ri}, {{nullptr, nullptr, nullptr, nullptr}}, "
+                           f"{{nullptr, nullptr, nullptr, nullptr}}, {{nullptr, nullptr}}}},")
+                continue
             kf = f"stockham_kernel<{n}, {t}, -1, {rs}>"
             ki = f"stockham_kernel<{n}, {t}, 1, {rs}>"
             lines.append(f"template __global__ void {kf}(const StockhamArgs);")
             lines.append(f"template __global__ void {ki}(const StockhamArgs);")
-            radix = ", ".join(str(r) for r in seq)
             if t >= 16:
                 rf = f"stockham_rows_kernel<{n}, {t}, -1, {rs}>"
                 ri = f"stockham_rows_kernel<{n}, {t}, 1, {rs}>"

@@ -74,6 +74,7 @@ struct StockhamArgs {
     float *out_im;         // split-plane output: real parts at (float *)out, imaginary parts here
     int32_t in_sep;        // the input is two float planes (element offsets as for complex)
     int32_t out_sep;       // the output is two float planes
+    int32_t nt_out;        // single-plane C2C rows: the last pass stores nontemporal
     FastDiv div_in_len;    // divides by in_len
     FastDiv div_out_len;   // divides by out_len
     FastDiv div_F;         // divides by F

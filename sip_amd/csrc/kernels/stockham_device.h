@@ -30,30 +30,29 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &d)
 
 __device__ __forceinline__ int lds_pad(int j) { return j + (j >> 4); }
 
-// Streaming HBM accesses of the direct passes (read once / written once).
-#ifndef SIP_NT
-#define SIP_NT 0
+// Streaming HBM accesses of the direct passes (read once / written once): nontemporal loads
+// (SIP_NT_LOAD) are a tuning switch, off by default; nontemporal stores are chosen per launch
+// (StockhamArgs::nt_out, gstore_nt).
+#ifndef SIP_NT_LOAD
+#define SIP_NT_LOAD 0
 #endif
 typedef float sip_vf2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ float2 gload(const float2 *p)
 {
-#if SIP_NT
+#if SIP_NT_LOAD
     const sip_vf2 v = __builtin_nontemporal_load(reinterpret_cast<const sip_vf2 *>(p));
     return make_float2(v.x, v.y);
 #else
     return *p;
 #endif
 }
-__device__ __forceinline__ void gstore(float2 *p, float2 v)
+__device__ __forceinline__ void gstore(float2 *p, float2 v) { *p = v; }
+__device__ __forceinline__ void gstore_nt(float2 *p, float2 v)
 {
-#if SIP_NT
     sip_vf2 w;
     w.x = v.x;
     w.y = v.y;
     __builtin_nontemporal_store(w, reinterpret_cast<sip_vf2 *>(p));
-#else
-    *p = v;
-#endif
 }
 
 template <int A, int... Rest>
@@ -139,14 +138,26 @@ struct RowIo {  // contiguous transform
     static constexpr bool kRegTail = false;
     const float2 *__restrict__ in;
     float2 *__restrict__ out;
+    bool nt = false;  // nontemporal stores (an output that no later launch of the plan reads)
     __device__ __forceinline__ float2 load(int j) const { return gload(in + j); }
-    __device__ __forceinline__ void store(int j, float2 v) const { gstore(out + j, v); }
+    __device__ __forceinline__ void store(int j, float2 v) const
+    {
+        if (nt)
+            gstore_nt(out + j, v);
+        else
+            gstore(out + j, v);
+    }
     // the R outputs d + r NS of one butterfly
     template <int R, int NS>
     __device__ __forceinline__ void store_bf(int d, const float2 (&v)[R]) const
     {
+        if (nt) {
 #pragma unroll
-        for (int r = 0; r < R; r++) gstore(out + d + r * NS, v[r]);
+            for (int r = 0; r < R; r++) gstore_nt(out + d + r * NS, v[r]);
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) gstore(out + d + r * NS, v[r]);
+        }
     }
 };
 
@@ -489,7 +500,7 @@ __device__ __forceinline__ void split_rows(const StockhamArgs &a, float *pl, int
                                              R2cTail{static_cast<float2 *>(a.out) + ob, a.rtw});
     else
         split_chain<L, T, S, 1, R0, Rest...>(pl, a.tw, flp * a.LP, t, active, v,
-                                             RowIo{nullptr, static_cast<float2 *>(a.out) + ob});
+                                             RowIo{nullptr, static_cast<float2 *>(a.out) + ob, a.nt_out != 0});
 }
 
 // C2R rows with one LDS plane: the half spectrum X[0..L] is read from HBM in pair units (unit kk > 0

@@ -85,11 +85,15 @@ def test_alone_equals_mix(torch_cuda):
             off = int(rng.integers(0, pool.numel() - 2 * bn * bb - 2)) // 2 * 2
             background.append((bt, bn, bb, off))
     torch.cuda.synchronize()
-    tickets = []
+    # one hand-over of the whole burst (the scheduler sees every request at once, so requests of
+    # one shape coalesce whatever the host's submission speed)
+    reqs = []
     for i, (t, n, b, x, xd, yd) in enumerate(cases):
-        tickets.append(srv.submit(t, n, b, xd.data_ptr(), yd.data_ptr()))
+        reqs.append((t, n, b, xd.data_ptr(), yd.data_ptr()))
         for bt, bn, bb, off in background[3 * i: 3 * i + 3]:
-            srv.submit(bt, bn, bb, pool.data_ptr() + 4 * off)
+            reqs.append((bt, bn, bb, pool.data_ptr() + 4 * off, None))
+    all_tickets = srv.submit_many(reqs)
+    tickets = all_tickets[0::4]
     for tk in tickets:
         st, lat = srv.wait(tk)
         assert st == 0 and lat > 0
@@ -119,13 +123,14 @@ def test_benchmark_modest_rate(torch_cuda):
 
 
 def test_submit_validation(torch_cuda):
-    """Requests refused at submit: C2R (not served), n = 0, an output beyond the budget, and a Bluestein
-    length whose workspace (86.6 MB for 10007 x 100) exceeds the 64 MiB budget although its
-    input and output would fit. The input is a real device allocation, so a request that passed
-    validation by mistake would still run in bounds."""
+    """Requests refused at submit: C2R (not served), n = 0, and a Bluestein length whose workspace
+    (157 MB for 20011 x 100: two 65536-point rows per transform) exceeds the 64 MiB budget although
+    its input and output would fit, with and without a caller output. The input is a real device
+    allocation, so a request that passed validation by mistake would still run in bounds."""
     torch = torch_cuda
     import ctypes
-    x = torch.zeros((100, 10007), dtype=torch.complex64, device="cuda")
+    n = 20011
+    x = torch.zeros((100, n), dtype=torch.complex64, device="cuda")
     y = torch.empty_like(x)
     srv = S.Server(budget_bytes=1 << 26, nstreams=1, max_coalesce_rows=64)
     t = ctypes.c_uint64()
@@ -133,13 +138,23 @@ def test_submit_validation(torch_cuda):
     xp = ctypes.c_void_p(x.data_ptr())
     assert L.sipFftServeSubmit(srv.h, 0x11, 1024, 1, xp, None, ctypes.byref(t)) != 0  # C2R
     assert L.sipFftServeSubmit(srv.h, 0x10, 0, 1, xp, None, ctypes.byref(t)) != 0
-    assert L.sipFftServeSubmit(srv.h, 0x10, 10007, 100, xp, None, ctypes.byref(t)) != 0   # output > budget
-    assert L.sipFftServeSubmit(srv.h, 0x10, 10007, 100, xp, ctypes.c_void_p(y.data_ptr()),
-                               ctypes.byref(t)) != 0                                     # workspace > budget
+    assert L.sipFftServeSubmit(srv.h, 0x10, n, 100, xp, None, ctypes.byref(t)) != 0   # output + workspace > budget
+    assert L.sipFftServeSubmit(srv.h, 0x10, n, 100, xp, ctypes.c_void_p(y.data_ptr()),
+                               ctypes.byref(t)) != 0                                 # workspace > budget
+    # a batched hand-over with one refused request queues none of them and names the refused one
+    with pytest.raises(S.SipFftError, match=r"request 1\)"):
+        srv.submit_many([(S.ASCEND_FFT_C2C, 1024, 1, x.data_ptr(), None),
+                         (S.ASCEND_FFT_C2C, n, 100, x.data_ptr(), None),
+                         (S.ASCEND_FFT_C2C, 1024, 1, x.data_ptr(), None)])
+    assert srv.drain() == 0 and srv.stats()["launches"] == 0
     # the same length at a batch whose workspace fits is served and correct
-    tk = srv.submit(S.ASCEND_FFT_C2C, 10007, 10, x.data_ptr(), y.data_ptr())
+    xs = oracle.uniform_complex((10, n), 31)
+    x[:10].copy_(torch.from_numpy(xs))
+    tk = srv.submit(S.ASCEND_FFT_C2C, n, 10, x.data_ptr(), y.data_ptr())
     st, _ = srv.wait(tk)
     assert st == 0
+    rel, _ = oracle.error(y[:10].cpu().numpy(), oracle.c2c(xs))
+    assert rel <= 5e-6, rel
     srv.destroy()
 
 

@@ -10,6 +10,8 @@ import re
 import subprocess
 import threading
 
+import numpy as np
+
 import pytest
 
 import oracle
@@ -221,6 +223,32 @@ def test_plan_layouts():
     assert len(d["stages"]) == 1 and d["stages"][0]["io"] == 1 and d["stages"][0]["L"] == 4096
     d = dry([100, 200], S.ASCEND_FFT_C2R, 2)
     assert d["workspace"] > 0  # the input is not modified, so the complex axis goes to workspace
+
+
+@pytest.mark.parametrize("n", [16875, 19683, 20000, 24576, 30375, 32768])
+def test_single_pass_long_rows(n):
+    """2,3,5-smooth rows above 16384 with a radix sequence of at most 32 values per thread, 256 or
+    more of them: one single-plane launch of 1024 threads per transform, no workspace; as the last
+    axis of a 2D transform as well, and for the half-length transform of an R2C row. Fewer rows keep
+    the four-step split."""
+    d = dry([n], S.ASCEND_FFT_C2C, 256)
+    assert d["workspace"] == 0 and len(d["stages"]) == 1
+    st = d["stages"][0]
+    assert st["L"] == n and st["F"] == 1 and st["split_lds"] == 1 and st["direct_in"] == st["direct_out"] == 1
+    assert int(np.prod(st["radices"])) == n and st["T"] <= 1024 and st["lds"] <= 160 * 1024
+    assert len(dry([n], S.ASCEND_FFT_C2C, 255)["stages"]) == 2
+    d = dry([6, n], S.ASCEND_FFT_C2C, 50)
+    assert d["stages"][0]["L"] == n and d["stages"][0]["nfft"] == 300
+    d = dry([2 * n], S.ASCEND_FFT_R2C, 300)
+    assert d["stages"][0]["L"] == n and d["stages"][0]["note"] == f"leaf {n}"
+
+
+@pytest.mark.parametrize("n", [31104, 32000, 65536, 3 ** 10])
+def test_long_rows_without_single_pass_sequence(n):
+    """Lengths with no sequence within 32 values per thread (31104, 32000) and lengths above 32768
+    keep the four-step split."""
+    d = dry([n], S.ASCEND_FFT_C2C, 512)
+    assert len(d["stages"]) >= 2 and d["stages"][0]["tw4"] == n
 
 
 def test_single_plane_kernel_choices():
