@@ -1071,13 +1071,20 @@ void plan_materialize_external(Plan &p, const std::vector<const int64_t *> &dev_
     plan_materialize(p);
 }
 
-// Nontemporal final stores for a stage whose output no later stage of the plan reads (the single-plane
-// C2C rows kernel honours it). Measured on MI355X, 4096 x 16384 C2C: 188 against 193 us per launch;
-// the same stores on four-step intermediates and column tiles cost more than they gained (batching
-// workload 9.1 -> 10.2 ms), so only this last-pass row store takes them.
+// Streaming (nontemporal) HBM accesses per launch. Stores: the last pass of a stage whose output no
+// later stage of the plan reads (the single-plane C2C rows kernel honours it). Loads: the first pass
+// of a stage that reads 512 MiB or more (single-plane rows, column tiles, rows-in / columns-out),
+// far beyond the 256 MB last-level cache. Measured on MI355X (one box, in-process A/B against plain
+// accesses): nontemporal stores 4096 x 16384 C2C 192 -> 184 us and the batching workload 9.2 ->
+// 9.0 ms, neutral elsewhere; nontemporal loads on top 186 -> 177 us, 1024 x 65536 186 -> 180 us,
+// 2D 1024^2 x 128 855 -> 837 us, 3D 128^3 x 64 801 -> 786 us, 32768 x 4096 540 -> 532 us, but
+// slower on inputs that the last-level cache still holds (2000 x 8192 45 -> 56 us, R2C 8192 x 4096
+// 48 -> 52 us, four-step 43200 x 1056 341 -> 359 us, batching workload 8.9 -> 9.3 ms), hence the
+// size threshold.
 void plan_mark_streaming(Plan &p)
 {
-    const bool on = env_flag("SIP_FFT_NT_OUT", true);
+    const bool on_out = env_flag("SIP_FFT_NT_OUT", true), on_in = env_flag("SIP_FFT_NT_IN", true);
+    constexpr int64_t kNtInBytes = int64_t(512) << 20;
     for (size_t i = 0; i < p.stages.size(); i++) {
         Stage &s = p.stages[i];
         if (s.kind != StageKind::Stockham) continue;
@@ -1087,7 +1094,9 @@ void plan_mark_streaming(Plan &p)
             if (in.kind == s.out.kind && (in.kind == BUF_IN || in.kind == BUF_OUT || in.ws_index == s.out.ws_index))
                 read_later = true;
         }
-        s.sa.nt_out = (on && !read_later) ? 1 : 0;
+        s.sa.nt_out = (on_out && !read_later) ? 1 : 0;
+        const int64_t in_bytes = s.sa.nfft * (int64_t)s.sa.in_len * 8;
+        s.sa.nt_in = (on_in && s.sa.io_mode == IO_C2C && in_bytes >= kNtInBytes) ? 1 : 0;
     }
 }
 
@@ -1239,7 +1248,7 @@ std::string plan_describe(const Plan &p)
                << ",\"direct_in\":" << s.sa.direct_in << ",\"direct_out\":" << s.sa.direct_out
                << ",\"split_lds\":" << s.sa.split_lds << ",\"col_direct\":" << s.sa.col_direct << ",\"rc\":" << (s.rc ? 1 : 0)
                << ",\"in_sep\":" << s.sa.in_sep << ",\"out_sep\":" << s.sa.out_sep << ",\"mfma\":" << (s.mfw.used() ? 1 : 0)
-               << ",\"xcd_remap\":" << s.sa.xcd_remap << ",\"nt_out\":" << s.sa.nt_out;
+               << ",\"xcd_remap\":" << s.sa.xcd_remap << ",\"nt_out\":" << s.sa.nt_out << ",\"nt_in\":" << s.sa.nt_in;
             if (s.inst2d) os << ",\"tile2d\":[" << s.inst2d->X << "," << s.inst2d->Y << "]";
         } else {
             os << ",\"mode\":" << s.xa.mode << ",\"rows\":" << s.xa.rows << ",\"n\":" << s.xa.n

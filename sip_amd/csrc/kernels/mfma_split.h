@@ -493,8 +493,9 @@ __device__ __forceinline__ void mf2_cols(const StockhamArgs &a, float *pl, int64
     mf2_chain<L, T, S, DT, NT, 1, R0, Rest...>(pl, a.tw, W, a.LP, flp * a.LP, t, active, ml, v, io);
 }
 
+// Column tiles with three BF16 terms: four waves per SIMD (128 VGPRs); at five they spill.
 template <int L, int T, int S, int DT, int NT, int... Rs>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(SIP_MF2_WAVES))) void stockham_mf2_cols_kernel(
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(NT == 3 ? 4 : SIP_MF2_WAVES))) void stockham_mf2_cols_kernel(
     const StockhamArgs a)
 {
     extern __shared__ float lds[];

@@ -30,21 +30,14 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &d)
 
 __device__ __forceinline__ int lds_pad(int j) { return j + (j >> 4); }
 
-// Streaming HBM accesses of the direct passes (read once / written once): nontemporal loads
-// (SIP_NT_LOAD) are a tuning switch, off by default; nontemporal stores are chosen per launch
-// (StockhamArgs::nt_out, gstore_nt).
-#ifndef SIP_NT_LOAD
-#define SIP_NT_LOAD 0
-#endif
+// Streaming HBM accesses of the direct passes (read once / written once): nontemporal loads and
+// stores are chosen per launch (StockhamArgs::nt_in / nt_out).
 typedef float sip_vf2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ float2 gload(const float2 *p)
+__device__ __forceinline__ float2 gload(const float2 *p) { return *p; }
+__device__ __forceinline__ float2 gload_nt(const float2 *p)
 {
-#if SIP_NT_LOAD
     const sip_vf2 v = __builtin_nontemporal_load(reinterpret_cast<const sip_vf2 *>(p));
     return make_float2(v.x, v.y);
-#else
-    return *p;
-#endif
 }
 __device__ __forceinline__ void gstore(float2 *p, float2 v) { *p = v; }
 __device__ __forceinline__ void gstore_nt(float2 *p, float2 v)
@@ -487,12 +480,23 @@ __device__ __forceinline__ void split_rows(const StockhamArgs &a, float *pl, int
     }
     const float2 *g = static_cast<const float2 *>(a.in) + ib;
     float2 v[NB0][R0];
+    if (a.nt_in) {
 #pragma unroll
-    for (int q = 0; q < NB0; q++) {
-        const int b = t + q * T;
-        if (active && (EXACT0 || b < NBF0)) {
+        for (int q = 0; q < NB0; q++) {
+            const int b = t + q * T;
+            if (active && (EXACT0 || b < NBF0)) {
 #pragma unroll
-            for (int r = 0; r < R0; r++) v[q][r] = gload(g + b + r * NBF0);
+                for (int r = 0; r < R0; r++) v[q][r] = gload_nt(g + b + r * NBF0);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < NB0; q++) {
+            const int b = t + q * T;
+            if (active && (EXACT0 || b < NBF0)) {
+#pragma unroll
+                for (int r = 0; r < R0; r++) v[q][r] = gload(g + b + r * NBF0);
+            }
         }
     }
     if constexpr (R2C)
@@ -699,12 +703,23 @@ __device__ __forceinline__ void split_cols(const StockhamArgs &a, float *pl, int
     const ColIoT<TW4 ? 1 : 0> io{static_cast<const float2 *>(a.in) + ib, static_cast<float2 *>(a.out) + ob,
                                  a.tin.es, a.tout.es, col, &a};
     float2 v[NB0][R0];
+    if (a.nt_in) {
 #pragma unroll
-    for (int q = 0; q < NB0; q++) {
-        const int b = t + q * T;
-        if (active && (EXACT0 || b < NBF0)) {
+        for (int q = 0; q < NB0; q++) {
+            const int b = t + q * T;
+            if (active && (EXACT0 || b < NBF0)) {
 #pragma unroll
-            for (int r = 0; r < R0; r++) v[q][r] = io.load(b + r * NBF0);
+                for (int r = 0; r < R0; r++) v[q][r] = gload_nt(io.in + (int64_t)(b + r * NBF0) * io.es_in);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < NB0; q++) {
+            const int b = t + q * T;
+            if (active && (EXACT0 || b < NBF0)) {
+#pragma unroll
+                for (int r = 0; r < R0; r++) v[q][r] = io.load(b + r * NBF0);
+            }
         }
     }
     split_chain<L, T, S, 1, R0, Rest...>(pl, a.tw, flp * a.LP, t, active, v, io);
@@ -745,12 +760,23 @@ __device__ __forceinline__ void split_rows_cols(const StockhamArgs &a, float *pl
     if (act2) ob = tile_base((uint32_t)f2, a.tout);
     const float2 *g = static_cast<const float2 *>(a.in) + ib;
     float2 v[NB0][R0];
+    if (a.nt_in) {
 #pragma unroll
-    for (int q = 0; q < NB0; q++) {
-        const int b = tr + q * T;
-        if (act1 && (EXACT0 || b < NBF0)) {
+        for (int q = 0; q < NB0; q++) {
+            const int b = tr + q * T;
+            if (act1 && (EXACT0 || b < NBF0)) {
 #pragma unroll
-            for (int r = 0; r < R0; r++) v[q][r] = gload(g + b + r * NBF0);
+                for (int r = 0; r < R0; r++) v[q][r] = gload_nt(g + b + r * NBF0);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < NB0; q++) {
+            const int b = tr + q * T;
+            if (act1 && (EXACT0 || b < NBF0)) {
+#pragma unroll
+                for (int r = 0; r < R0; r++) v[q][r] = gload(g + b + r * NBF0);
+            }
         }
     }
     const ColIoT<0> io{nullptr, static_cast<float2 *>(a.out) + ob, 0, a.tout.es, 0, &a};

@@ -251,6 +251,21 @@ def test_long_rows_without_single_pass_sequence(n):
     assert len(d["stages"]) >= 2 and d["stages"][0]["tw4"] == n
 
 
+def test_streaming_access_choices():
+    """Nontemporal stores on a stage whose output no later stage reads; nontemporal loads on C2C
+    stages that read 512 MiB or more."""
+    st = dry([4096], S.ASCEND_FFT_C2C, 16384)["stages"][0]          # 512 MiB in
+    assert st["nt_out"] == 1 and st["nt_in"] == 1
+    st = dry([4096], S.ASCEND_FFT_C2C, 16383)["stages"][0]          # just below
+    assert st["nt_out"] == 1 and st["nt_in"] == 0
+    d = dry([1024, 1024], S.ASCEND_FFT_C2C, 8)                      # rows then columns in place
+    assert [s["nt_out"] for s in d["stages"]] == [0, 1] and [s["nt_in"] for s in d["stages"]] == [0, 0]
+    d = dry([65536], S.ASCEND_FFT_C2C, 2048)                        # four-step, 1 GiB per stage
+    assert [s["nt_out"] for s in d["stages"]] == [0, 1] and [s["nt_in"] for s in d["stages"]] == [1, 1]
+    st = dry([8192], S.ASCEND_FFT_R2C, 1 << 16)["stages"][0]        # real rows keep cached loads
+    assert st["nt_in"] == 0
+
+
 def test_single_plane_kernel_choices():
     """The launch shapes the measured defaults rely on: one LDS plane for contiguous C2C rows, the
     fused real-transform rows, column tiles (with the four-step twiddle on store for stage A) and the
