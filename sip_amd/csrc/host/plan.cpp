@@ -353,7 +353,13 @@ bool env_flag(const char *name, bool dflt)
 
 constexpr int64_t kColumnLeafMax = 1024;
 constexpr int64_t kRowLeafMax = 16384;
-constexpr int64_t kBigLeafMinRows = 256;  // single-pass leaves above kRowLeafMax: transforms per launch
+// single-pass leaves above kRowLeafMax: transforms per launch from which they replace the split
+// (tuning switch SIP_FFT_BIG_LEAF_ROWS)
+static int64_t big_leaf_min_rows()
+{
+    const char *e = std::getenv("SIP_FFT_BIG_LEAF_ROWS");
+    return (e && e[0]) ? std::atoll(e) : 256;
+}
 constexpr int64_t kFillWorkgroups = 512;  // two workgroups per CU of MI355X (256 CUs)
 
 struct Builder {
@@ -397,7 +403,7 @@ struct Builder {
         // 1.39 ms, 30375 x 4418 1.11 / 1.55 ms; with few transforms the split's parallelism wins
         // (32768 x 16 23 / 14 us, 20000 x 64 25 / 20 us).
         return inst->rows_fwd != nullptr && in.es == 1 && out.es == 1 && !is_user(in.buf) && !is_user(out.buf) &&
-               p.precision == PREC_FP32 && count(normalize(in.dims)) >= kBigLeafMinRows &&
+               p.precision == PREC_FP32 && count(normalize(in.dims)) >= big_leaf_min_rows() &&
                env_flag("SIP_FFT_BIG_LEAF", true);
     }
 
@@ -1094,9 +1100,17 @@ void plan_mark_streaming(Plan &p)
             if (in.kind == s.out.kind && (in.kind == BUF_IN || in.kind == BUF_OUT || in.ws_index == s.out.ws_index))
                 read_later = true;
         }
-        s.sa.nt_out = (on_out && !read_later) ? 1 : 0;
+        // Nontemporal accesses only where every wave covers whole 128-byte lines: a run of T
+        // consecutive complex values per radix slot, at multiples of 16 elements (2000 x 8192 with
+        // T = 100 wrote 7 % more bytes with nontemporal stores); column tiles 16 transforms wide.
+        const StockhamInstance *in = s.inst;
+        const bool rows_aligned_in = in && in->T % 16 == 0 && (in->L / in->radix[0]) % 16 == 0;
+        const bool rows_aligned_out = in && in->T % 16 == 0 && (in->L / in->radix[in->npass - 1]) % 16 == 0;
+        const bool cols_in = s.sa.col_direct && s.sa.split_lds;
+        const bool aligned_in = cols_in ? s.sa.F % 16 == 0 : rows_aligned_in;
+        s.sa.nt_out = (on_out && !read_later && rows_aligned_out) ? 1 : 0;
         const int64_t in_bytes = s.sa.nfft * (int64_t)s.sa.in_len * 8;
-        s.sa.nt_in = (on_in && s.sa.io_mode == IO_C2C && in_bytes >= kNtInBytes) ? 1 : 0;
+        s.sa.nt_in = (on_in && s.sa.io_mode == IO_C2C && in_bytes >= kNtInBytes && aligned_in) ? 1 : 0;
     }
 }
 

@@ -264,6 +264,10 @@ def test_streaming_access_choices():
     assert [s["nt_out"] for s in d["stages"]] == [0, 1] and [s["nt_in"] for s in d["stages"]] == [1, 1]
     st = dry([8192], S.ASCEND_FFT_R2C, 1 << 16)["stages"][0]        # real rows keep cached loads
     assert st["nt_in"] == 0
+    st = dry([2000], S.ASCEND_FFT_C2C, 1 << 16)["stages"][0]        # T = 100: runs not whole lines
+    assert st["T"] == 100 and st["nt_out"] == 0 and st["nt_in"] == 0
+    st = dry([1536], S.ASCEND_FFT_C2C, 1 << 16)["stages"][0]        # T = 96, radix-16 first / radix-6 last
+    assert st["T"] == 96 and st["nt_out"] == 1 and st["nt_in"] == 1
 
 
 def test_single_plane_kernel_choices():
