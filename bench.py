@@ -14,7 +14,8 @@ profiles/traffic_c2c_4096x16384.json when present.
 cpu_baseline: the oracle's FP32 CPU FFT (same staged Stockham algorithm) on the host cores (every core
 the process may use, at most 16), median of runs over the full 16384 x 4096 batch, and the FP64
 oracle FFT on 4096 of the rows; the leg also checks a sample of the GPU's output rows against the
-FP64 oracle. Secondary configurations (R2C/C2R 8192 x 4096, C2C 1536 / 2000 x 8192, C2C 1024 x 256,
+FP64 oracle, and times scipy.fft (pocketfft) on the same threads as a comparator when installed.
+Secondary configurations (R2C/C2R 8192 x 4096, C2C 1536 / 2000 x 8192, C2C 1024 x 256,
 2D 1024^2 x 8; and 1 GiB arrays beyond the last-level cache: four-step 65536 x 2048, 2D 1024^2 x 128,
 3D 128^3 x 64) are timed the same way and reported under "secondary"; the precision, batching and
 serving forms have legs of their own.
@@ -460,6 +461,23 @@ def main():
                                "fp64_value": round(fl64 / t64m / 1e9, 2), "fp64_seconds_4096_rows": round(t64m, 3),
                                "gpu_rel_l2_vs_fp64_sample": rel, "gpu_max_abs_vs_fp64_sample": mx,
                                "gpu_check_rows": int(rows.size)}
+        # extra comparator: pocketfft (scipy.fft, single precision for complex64) on the same threads
+        try:
+            import scipy.fft as sfft
+        except ImportError:
+            sfft = None
+        if sfft is not None:
+            sfft.fft(xs[:256], axis=-1, workers=threads)
+            ts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                sfft.fft(xs, axis=-1, workers=threads)
+                ts.append(time.perf_counter() - t0)
+            tsm = sorted(ts)[1]
+            out["cpu_baseline"].update({"scipy_fft_value": round(flops / tsm / 1e9, 2),
+                                        "scipy_fft_seconds_per_batch": round(tsm, 3),
+                                        "scipy_fft_note": "comparator only: scipy.fft (pocketfft) complex64, "
+                                                          "workers = `cores`, full batch, median of 3"})
     h.destroy()
     print(json.dumps(out), flush=True)
 
