@@ -135,7 +135,9 @@ int32_t sipFftBatchDestroy(void *handle);
  * index of the refused request), then they enter the queue together, so those of one shape can
  * coalesce; tickets (may be NULL) receives one ticket per request. Wait blocks until the request finished and
  * returns its status and latency (arrival to observed completion). Stats returns JSON
- * (p50/p90/p99 latency, launches, coalescing, arena use); with reset = 1 the counters are cleared
+ * (p50/p90/p99 latency, launches, coalescing, arena use, resident coefficient tables and their
+ * evictions: the tables are kept on the device outside the arena, up to a sixteenth of the budget
+ * or at least 64 MiB, least recently used dropped first); with reset = 1 the counters are cleared
  * only when the report fits `cap` (a size query with buf = NULL clears nothing). Benchmark runs an
  * open-loop Poisson stream (see serving.cpp) once per call and writes its JSON report into buf: pass
  * a buffer of 64 KiB (a short buffer loses the report and returns 100000 with *needed set). Calls

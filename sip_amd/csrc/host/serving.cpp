@@ -138,6 +138,10 @@ public:
             streams_.push_back(s);
         }
         inflight_per_stream_.assign((size_t)nstreams_, 0);
+        // resident coefficient tables live outside the arena: at most a sixteenth of the budget
+        // (at least 64 MiB), or SIP_FFT_SERVE_RESIDENT_MB
+        resident_cap_ = std::max<size_t>(size_t(64) << 20, budget / 16);
+        if (const char *e = std::getenv("SIP_FFT_SERVE_RESIDENT_MB")) resident_cap_ = (size_t)std::atoll(e) << 20;
         lat_ms_.reserve(size_t(1) << 22);  // a 60 s stream at 50,000 req/s without regrowing under the lock
         worker_ = std::thread([this] { loop(); });
     }
@@ -273,7 +277,9 @@ public:
            << ",\"coalesced_requests\":" << coalesced_ << ",\"queued\":" << queue_.size()
            << ",\"inflight_launches\":" << inflight_.size() << ",\"arena_used\":" << arena_->used()
            << ",\"arena_peak\":" << arena_peak_ << ",\"budget\":" << budget_ << ",\"bytes_done\":" << bytes_done_
-           << ",\"flops_done\":" << flops_done_ << "}";
+           << ",\"flops_done\":" << flops_done_ << ",\"resident_tables\":" << resident_.size()
+           << ",\"resident_bytes\":" << resident_bytes_ << ",\"resident_cap\":" << resident_cap_
+           << ",\"resident_evictions\":" << resident_evictions_ << "}";
         if (reset && os.str().size() + 1 <= cap) {
             lat_ms_.clear();
             launches_ = coalesced_ = mixed_launches_ = 0;
@@ -719,16 +725,34 @@ private:
         return true;
     }
 
-    // Every coefficient table a launch has used stays on the device for the server's lifetime: a
-    // later launch reuses it without a rebuild or an upload, and no table is freed (a
-    // device-synchronising hipFree) while launches run. Tables are held one by one, not per shape:
-    // launches of one (type, n) can differ in plan (a single-pass leaf from 256 rows up, a four-step
-    // split below), and each variant's tables must stay.
+    // Every coefficient table a launch has used stays on the device: a later launch reuses it
+    // without a rebuild or an upload, and no table is freed (a device-synchronising hipFree) while
+    // launches run. Tables are held one by one, not per shape: launches of one (type, n) can differ
+    // in plan (a single-pass leaf from 256 rows up, a four-step split below), and each variant's
+    // tables must stay. The resident set is capped (resident_cap_): beyond it the least recently used
+    // tables are dropped (a launch in flight keeps its own reference until it retires).
     void keep_resident(const Plan &plan)
     {
+        const uint64_t now = ++resident_clock_;
         for (const Stage &st : plan.stages)
-            for (const TableRef *t : {&st.tw, &st.rtw, &st.tw4lo, &st.tw4hi, &st.mfw, &st.tab0, &st.tw2})
-                if (t->dev) resident_.emplace(t->dev.get(), t->dev);
+            for (const TableRef *t : {&st.tw, &st.rtw, &st.tw4lo, &st.tw4hi, &st.mfw, &st.tab0, &st.tw2}) {
+                if (!t->dev) continue;
+                auto ins = resident_.emplace(t->dev.get(), Resident{t->dev, now});
+                if (ins.second)
+                    resident_bytes_ += t->dev->bytes;
+                else
+                    ins.first->second.last = now;
+            }
+        while (resident_bytes_ > resident_cap_) {
+            auto victim = resident_.end();
+            for (auto it = resident_.begin(); it != resident_.end(); ++it)
+                if (it->second.last != now && (victim == resident_.end() || it->second.last < victim->second.last))
+                    victim = it;
+            if (victim == resident_.end()) break;  // only the current launch's tables are left
+            resident_bytes_ -= victim->second.table->bytes;
+            resident_.erase(victim);
+            resident_evictions_++;
+        }
     }
 
     size_t budget_;
@@ -743,7 +767,13 @@ private:
     }();
     std::map<int64_t, bool> mixed_shape_;  // n -> runs in persistent launches
     uint64_t mixed_launches_ = 0;
-    std::unordered_map<const void *, TablePtr> resident_;
+    struct Resident {
+        TablePtr table;
+        uint64_t last = 0;  // resident_clock_ of the latest launch that used it
+    };
+    std::unordered_map<const void *, Resident> resident_;
+    size_t resident_bytes_ = 0, resident_cap_ = 0;
+    uint64_t resident_clock_ = 0, resident_evictions_ = 0;
     int64_t max_rows_;
     int device_ = 0;
     void *arena_base_ = nullptr;

@@ -268,6 +268,44 @@ def test_admission_waits_for_memory(torch_cuda):
     srv.destroy()
 
 
+def test_resident_tables_capped(torch_cuda, monkeypatch):
+    """With a zero cap on the resident coefficient tables (only the current launch's tables stay),
+    six shapes (four-step, single-pass long rows, Bluestein, R2C) served twice in turn: the other
+    shapes' tables are dropped and rebuilt when their shape returns, and every result equals the
+    first round's bit for bit and the request run alone through the plain API. (All six shapes'
+    tables together take under 1 MiB: the default cap of a sixteenth of the budget, at least
+    64 MiB, holds thousands of shapes.)"""
+    torch = torch_cuda
+    monkeypatch.setenv("SIP_FFT_SERVE_RESIDENT_MB", "0")
+    srv = S.Server(budget_bytes=1 << 30, nstreams=2, max_coalesce_rows=1024)
+    shapes = [(S.ASCEND_FFT_C2C, 65536, 4), (S.ASCEND_FFT_C2C, 20000, 300), (S.ASCEND_FFT_C2C, 10007, 4),
+              (S.ASCEND_FFT_R2C, 32768, 4), (S.ASCEND_FFT_C2C, 6000, 8), (S.ASCEND_FFT_C2C, 3001, 4)]
+    inputs = [_watched_inputs(torch, t, n, b, 8200 + k) for k, (t, n, b) in enumerate(shapes)]
+    torch.cuda.synchronize()
+    first = []
+    for rnd in range(2):
+        for k, ((t, n, b), (x, xd, yd)) in enumerate(zip(shapes, inputs)):
+            y = torch.empty_like(yd)
+            st, _ = srv.wait(srv.submit(t, n, b, xd.data_ptr(), y.data_ptr()))
+            assert st == 0
+            if rnd == 0:
+                first.append(y)
+            else:
+                assert torch.equal(first[k], y), f"shape {k}: second round differs after eviction"
+    stats = srv.stats()
+    print(f"resident: {stats['resident_tables']} tables, {stats['resident_bytes'] / 2**20:.2f} MiB, "
+          f"{stats['resident_evictions']} evictions", flush=True)
+    assert stats["resident_cap"] == 0 and stats["resident_evictions"] > 0
+    assert stats["resident_bytes"] < 1 << 20
+    for k, ((t, n, b), (x, xd, yd)) in enumerate(zip(shapes, inputs)):
+        plain = _alone(torch, t, n, b, xd)
+        rel, _ = oracle.error(first[k].cpu().numpy(), plain.cpu().numpy().astype(np.complex128))
+        assert rel <= 5e-7, f"shape {k}: served vs plain API rel-L2 {rel:.3e}"
+        if n not in (10007, 3001):   # Bluestein lengths are held to the plain API's result above
+            _check_rows(f"resident shape {k}", t, n, x, first[k])
+    srv.destroy()
+
+
 def test_coalesced_aux_lengths(torch_cuda):
     """Coalesced launches of lengths whose plans carry auxiliary stages over the gathered rows
     (1, 17 and 997: copy and Bluestein; 20000: four-step): every result equals the request alone."""
