@@ -468,11 +468,17 @@ private:
                 head->status = AsdSip::ACL_ERROR_INTERNAL_ERROR;
                 head->t_done = now_s();
                 queue_.pop_front();
+                completed_err_++;
+                if (head->detached) reqs_.erase(head->id);
                 done_cv_.notify_all();
                 any = true;
                 continue;
             }
-            for (Req *r : group) queue_.erase(std::find(queue_.begin(), queue_.end(), r));
+            for (Req *r : group) {
+                queue_.erase(std::find(queue_.begin(), queue_.end(), r));
+                // a launch that failed has already completed its requests (with an error status)
+                if (r->done && r->detached) reqs_.erase(r->id);
+            }
             any = true;
         }
         return any;
@@ -600,6 +606,7 @@ private:
                 r->done = true;
                 r->status = AsdSip::ACL_ERROR_INTERNAL_ERROR;
                 r->t_done = now_s();
+                completed_err_++;
             }
             done_cv_.notify_all();
             return true;
@@ -711,6 +718,7 @@ private:
                 r->done = true;
                 r->status = AsdSip::ACL_ERROR_INTERNAL_ERROR;
                 r->t_done = now_s();
+                completed_err_++;
             }
             done_cv_.notify_all();
             return true;
