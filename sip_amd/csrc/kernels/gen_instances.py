@@ -201,8 +201,13 @@ def instances():
 # single-plane column-tile kernel
 COLUMN_LEAF_MAX = 1024
 # half lengths (with 32 or more threads per transform) of the real transforms that also get
-# single-plane R2C / C2R row kernels
+# single-plane R2C / C2R row kernels: every one up to 8192, and those above 8192 where the single
+# plane measured faster than the two-plane fused kernel on MI355X (R2C + C2R of 256 MiB of reals,
+# one box: 9216 155 / 211 us, 10000 161 / 226, 16384 131 / 173; it lost for most mixed lengths from
+# 12000 up, e.g. 15000 241 / 218, 13824 225 / 212)
 REAL_ROWS_MAX = 8192
+REAL_ROWS_LONG = {8640, 9000, 9216, 9375, 9600, 9720, 10000, 10125, 10240, 10368, 10800, 10935, 11250, 11664,
+                  12288, 16384}
 # lengths up to COLUMN_LEAF_MAX with two or more passes and at least this many threads per
 # transform (64-byte runs per row read) also get the rows-in / columns-out kernel of the second
 # four-step stage
@@ -261,7 +266,7 @@ def main():
                 cols = "{" + ", ".join(f"(const void *)&{nm}" for nm in names) + "}"
             else:
                 cols = "{nullptr, nullptr, nullptr, nullptr}"
-            if t >= 32 and n <= REAL_ROWS_MAX:
+            if t >= 32 and (n <= REAL_ROWS_MAX or n in REAL_ROWS_LONG):
                 names = [f"stockham_real_rows_kernel<{n}, {t}, {sg}, {m}, {rs}>" for m in (1, 2) for sg in (-1, 1)]
                 for nm in names:
                     lines.append(f"template __global__ void {nm}(const StockhamArgs);")

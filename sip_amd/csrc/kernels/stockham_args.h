@@ -107,8 +107,8 @@ struct StockhamInstance {
     const void *rows_inv;
     const void *cols[4];     // single-plane column-tile kernels, fwd / inv without and fwd / inv
                              // with the four-step twiddle (L <= 1024 only, else null)
-    const void *real_rows[4];  // single-plane R2C fwd / inv, C2R fwd / inv rows (L <= 8192 with
-                               // T >= 32 only, else null)
+    const void *real_rows[4];  // single-plane R2C fwd / inv, C2R fwd / inv rows (T >= 32; L <= 8192
+                               // and the longer L where they measured faster; else null)
     const void *rc[2];         // single-plane rows-in / columns-out fwd / inv (four-step second
                                // stage; L <= 1024 with two or more passes and T >= 8, else null)
 };

@@ -117,8 +117,8 @@ def test_c2c_2_pow_27(torch_cuda):
     assert rel <= TOL
 
 
-@pytest.mark.parametrize("n", [4, 6, 8, 10, 16, 126, 128, 256, 500, 1024, 2000, 4096, 8192, 16384, 32768,
-                               65536, 3, 5, 7, 9, 15, 243, 567, 997, 2187, 199 * 199, 15000])
+@pytest.mark.parametrize("n", [4, 6, 8, 10, 16, 126, 128, 256, 500, 1024, 2000, 4096, 8192, 16384, 18432, 20000,
+                               24000, 24576, 32768, 65536, 3, 5, 7, 9, 15, 243, 567, 997, 2187, 199 * 199, 15000])
 def test_r2c_c2r_lengths(torch_cuda, n):
     b = max(1, min(32, (1 << 19) // n))
     x = oracle.uniform(n * b, 5000 + n).reshape(b, n)
