@@ -197,6 +197,13 @@ def instances():
     return res
 
 
+# Lengths whose contiguous C2C rows run faster on the two-plane tile than on the single-plane rows
+# kernel (no rows kernel is emitted for them, so the planner and the tuner keep the tile). Measured
+# on MI355X, 256 MiB of rows, single / two planes: 12000 208 / 179 us, 15000 189 / 164, 15625
+# 288 / 240, 16200 233 / 201, 14580 190 / 168, 12150 205 / 181, 8748 141 / 134, 12960 173 / 164,
+# 15552 169 / 158; every other length from 4320 to 16384 is faster on one plane (up to 1.7x).
+ROWS_TWO_PLANE = {8748, 12000, 12150, 12960, 14580, 15000, 15552, 15625, 16200}
+
 # longest leaf used for column tiles (the host's kColumnLeafMax): these lengths also get the
 # single-plane column-tile kernel
 COLUMN_LEAF_MAX = 1024
@@ -250,7 +257,7 @@ def main():
             ki = f"stockham_kernel<{n}, {t}, 1, {rs}>"
             lines.append(f"template __global__ void {kf}(const StockhamArgs);")
             lines.append(f"template __global__ void {ki}(const StockhamArgs);")
-            if t >= 16:
+            if t >= 16 and n not in ROWS_TWO_PLANE:
                 rf = f"stockham_rows_kernel<{n}, {t}, -1, {rs}>"
                 ri = f"stockham_rows_kernel<{n}, {t}, 1, {rs}>"
                 lines.append(f"template __global__ void {rf}(const StockhamArgs);")
