@@ -658,20 +658,56 @@ struct Builder {
     }
 
     // Two-leaf split L = L1 * L2 for a four-step pair. Stage B stores transposed (a column tile),
-    // so both factors must be column leaves; the most balanced pair is taken.
+    // so both factors must be column leaves. Among the pairs within 4x of balanced the choice is, in
+    // this order:
+    //  1. HBM line alignment: stage A reads and writes runs of 16 adjacent columns at a stride of L2
+    //     elements and stage B writes them at a stride of L1, so a factor that is a multiple of 16
+    //     (8) keeps those runs on 128-byte lines (half lines); the smaller of the two factors'
+    //     alignments counts, and below 8 elements all count alike;
+    //  2. fewer radix passes in total;
+    //  3. balance (|log L1 - log L2|), then the smaller L1.
+    // Measured on MI355X over every 2,3,5-smooth length from 32805 to 65536 (128 MiB each), against
+    // the most balanced pair: 62208 = 144 x 432 100 us against 243 x 256 128 us, 64000 = 160 x 400
+    // against 250 x 256 (1.24x), 55296 1.20x, 34560 1.19x, 41472 1.16x, 34992 1.14x, 38400 1.13x;
+    // the balance cap and the smaller-L1 tie-break keep the rest where the most balanced pair was.
+    static int line_align(int64_t n)
+    {
+        int a = 1;
+        while (a < 16 && n % (2 * a) == 0) a *= 2;
+        return a < 8 ? 0 : a;
+    }
+
     static bool leaf_split(int64_t L, int64_t &L1, int64_t &L2)
     {
         L1 = L2 = 0;
-        double best = 1e30;
-        for (int64_t a = 2; a <= kColumnLeafMax && a < L; a++) {
-            if (L % a) continue;
-            const int64_t b = L / a;
-            if (b > kColumnLeafMax || !find_instance(a) || !find_instance(b)) continue;
-            const double score = std::fabs(std::log((double)a) - std::log((double)b));
-            if (score < best) {
-                best = score;
-                L1 = a;
-                L2 = b;
+        int best_align = -1, best_pass = 0;
+        double best_bal = 0;
+        for (int pass = 0; pass < 2 && L1 == 0; pass++) {
+            // second round without the balance cap (lengths with no pair within 4x)
+            const double cap = pass == 0 ? std::log(4.0) + 1e-9 : 1e30;
+            for (int64_t a = 2; a <= kColumnLeafMax && a < L; a++) {
+                if (L % a) continue;
+                const int64_t b = L / a;
+                const StockhamInstance *ia = find_instance(a), *ib = find_instance(b);
+                if (b > kColumnLeafMax || !ia || !ib) continue;
+                const double bal = std::fabs(std::log((double)a) - std::log((double)b));
+                if (bal > cap) continue;
+                const int align = std::min(line_align(a), line_align(b));
+                const int passes = ia->npass + ib->npass;
+                bool better;
+                if (align != best_align)
+                    better = align > best_align;
+                else if (passes != best_pass)
+                    better = passes < best_pass;
+                else
+                    better = bal < best_bal - 1e-9;  // ties keep the smaller L1 (found first)
+                if (better) {
+                    best_align = align;
+                    best_pass = passes;
+                    best_bal = bal;
+                    L1 = a;
+                    L2 = b;
+                }
             }
         }
         return L1 != 0;
@@ -710,9 +746,13 @@ struct Builder {
         }
         // Split L = L1 * L2: L1 column FFTs (+ twiddle), then L2 row FFTs with transposed store.
         int64_t L1 = 0, L2 = 0;
-        if (depth == 0 && p.tune.split_l1 > 1 && L % p.tune.split_l1 == 0 && p.tune.split_l1 <= kColumnLeafMax &&
-            find_instance(p.tune.split_l1) && decomposable(L / p.tune.split_l1, depth + 1)) {
-            L1 = p.tune.split_l1;
+        int64_t forced = p.tune.split_l1;
+        if (forced <= 1) {
+            if (const char *e = std::getenv("SIP_FFT_SPLIT_L1")) forced = std::atoll(e);  // tuning switch
+        }
+        if (depth == 0 && forced > 1 && L % forced == 0 && forced <= kColumnLeafMax && find_instance(forced) &&
+            decomposable(L / forced, depth + 1)) {
+            L1 = forced;
             L2 = L / L1;
         } else if (!leaf_split(L, L1, L2)) {
             for (int64_t l1 = std::min(L, kColumnLeafMax); l1 >= 2; l1--)
