@@ -403,7 +403,7 @@ struct Builder {
         // 1.39 ms, 30375 x 4418 1.11 / 1.55 ms; with few transforms the split's parallelism wins
         // (32768 x 16 23 / 14 us, 20000 x 64 25 / 20 us).
         return inst->rows_fwd != nullptr && in.es == 1 && out.es == 1 && !is_user(in.buf) && !is_user(out.buf) &&
-               p.precision == PREC_FP32 && count(normalize(in.dims)) >= big_leaf_min_rows() &&
+               p.precision == PREC_FP32 && p.big_leaf && count(normalize(in.dims)) >= big_leaf_min_rows() &&
                env_flag("SIP_FFT_BIG_LEAF", true);
     }
 

@@ -118,6 +118,10 @@ struct Plan {
     std::vector<const int64_t *> external_tables;  // caller-owned device row tables (serving layer)
     Tune tune;
     int precision = 0;  // PrecisionMode of the radix-16 stages of C2C launches
+    // Single-pass leaves for lengths 16384..32768 from 256 transforms up. A caller whose results
+    // must not depend on how many rows share a launch (the serving form's coalescing) turns them
+    // off: the four-step split is then taken at every row count.
+    bool big_leaf = true;
     // ISTFT plan (asdFftIstftMakePlan): inverse C2C of the (channel, n_fft, frames) spectra into a
     // frame workspace, then the windowed overlap-add into (channel, out_len)
     struct Istft {

@@ -311,6 +311,7 @@ private:
             Plan dry;
             dry.fftType = type;
             dry.direction = kDirForward;
+            dry.big_leaf = false;  // as the launches below
             dry.sizes = {n};
             dry.batch = batch;
             plan_build(dry);
@@ -515,6 +516,9 @@ private:
         }
         auto plan = std::make_unique<Plan>();
         plan->direction = kDirForward;
+        // a request's plan must not depend on how many rows it shares a launch with (results are
+        // bit-identical alone and coalesced), so no row-count dependent single-pass leaves
+        plan->big_leaf = false;
         // one request: its rows are contiguous, so the plain plan runs it without row tables (no
         // table staging, no host-to-device copy); the same kernels as the gathered plan
         const bool gathered = gathered_supported(type, n) && group.size() > 1;
@@ -735,10 +739,10 @@ private:
 
     // Every coefficient table a launch has used stays on the device: a later launch reuses it
     // without a rebuild or an upload, and no table is freed (a device-synchronising hipFree) while
-    // launches run. Tables are held one by one, not per shape: launches of one (type, n) can differ
-    // in plan (a single-pass leaf from 256 rows up, a four-step split below), and each variant's
-    // tables must stay. The resident set is capped (resident_cap_): beyond it the least recently used
-    // tables are dropped (a launch in flight keeps its own reference until it retires).
+    // launches run. Tables are held one by one, not per shape, so that every table any launch of a
+    // shape used stays (a single request and a coalesced group build separate plans). The resident
+    // set is capped (resident_cap_): beyond it the least recently used tables are dropped (a launch
+    // in flight keeps its own reference until it retires).
     void keep_resident(const Plan &plan)
     {
         const uint64_t now = ++resident_clock_;

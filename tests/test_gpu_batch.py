@@ -1,7 +1,9 @@
 """Batching form (sipFftBatch*): uneven C2C problems in one launch.
 
 Checks: every problem against the FP64 oracle (rel-L2 <= 2e-6); the batched result is bit-identical
-to each problem run alone through asdFftExecC2C (same kernels, same per-transform arithmetic);
+to each problem run alone through asdFftExecC2C when both take the same plan (same kernels, same
+per-transform arithmetic; the problems here keep every group of lengths 16384..32768 below the 256
+rows from which such a group runs as a single-pass leaf instead of a four-step split);
 hipGraph replays (with eager work and synchronisation in between) reproduce the eager result bit
 for bit; autotuned plans stay within tolerance and reuse the on-disk cache.
 """
