@@ -51,9 +51,11 @@ def factor(n):
 
 # Radix sequences chosen by measurement where the cost model's first choice measured slower: tuning
 # builds took the second and third sequence by cost (same pass count) for every length above 1024
-# where one exists (118 lengths), each timed as C2C rows of 128 MiB on MI355X, best of two runs per
-# build; kept where the alternative was faster by more than 5 % (run-to-run spread: median 0.6 %,
-# at most 3.4 %). Old / new microseconds in the comments.
+# where one exists (118 lengths), then the fourth and the best with one pass more (165 lengths),
+# each timed as C2C rows of 128 MiB on MI355X, best of two runs per build; kept where the
+# alternative was faster by more than 5 % (run-to-run spread: median 0.6 %, at most 3.4 % in the
+# first round, 6.2 % in the second, whose kept gains are all above 10 %). Old / new microseconds in
+# the comments.
 SEQ_OVERRIDE = {
     1120: [16, 10, 7],       # 51.6 / 46.6
     1200: [20, 12, 5],       # 49.6 / 46.6
@@ -78,6 +80,10 @@ SEQ_OVERRIDE = {
     11520: [16, 16, 9, 5],   # 66.6 / 54.6
     13122: [9, 9, 9, 6, 3],  # 78.5 / 73.3
     13824: [16, 16, 9, 6],   # 84.9 / 72.4
+    # a second round (the fourth sequence by cost, and the best one with one pass more)
+    2160: [12, 12, 5, 3],    # 51.6 / 46.5
+    14400: [20, 12, 12, 5],  # 66.2 / 57.1
+    15360: [20, 12, 4, 4, 4],  # 78.7 / 69.2
 }
 
 # Radices with a register butterfly (butterflies.h): prime, power-of-two and two-factor composites.
@@ -130,7 +136,13 @@ def radix_sequence(n):
             ranked.sort(key=lambda cs: cs[0])  # stable: ties keep the larger leading radices
             # tuning builds (SIP_GEN_SEQ_RANK=k) take the k-th sequence by cost among those with as
             # few passes as the first, for lengths above 1024
-            same = [cs for cs in ranked if len(cs[1]) == len(ranked[0][1])]
+            npass = len(ranked[0][1])
+            if n > 1024 and os.environ.get("SIP_GEN_SEQ_EXTRA") == "1":
+                # tuning builds: the best sequence with one pass more, where one exists
+                more = [cs for cs in ranked if len(cs[1]) == npass + 1]
+                if more:
+                    return more[0][1]
+            same = [cs for cs in ranked if len(cs[1]) == npass]
             rank = int(os.environ.get("SIP_GEN_SEQ_RANK", "0")) if n > 1024 else 0
             return same[min(rank, len(same) - 1)][1]
     f = factor(n)
