@@ -84,7 +84,12 @@ SEQ_OVERRIDE = {
     2160: [12, 12, 5, 3],    # 51.6 / 46.5
     14400: [20, 12, 12, 5],  # 66.2 / 57.1
     15360: [20, 12, 4, 4, 4],  # 78.7 / 69.2
+    # leaves up to 1024, timed in both four-step roles (n = 256 L, 128 MiB), as rows and as 2D L x L
+    720: [12, 12, 5],        # four-step 268 / 237, rows 51.9 / 45.2, 2D 120 / 101
+    800: [20, 8, 5],         # four-step 273 / 257, rows 46.1 / 48.8, 2D 124 / 109
 }
+# (1000 = 10 x 10 x 10 was faster in the four-step roles, 305 / 283 us, but its 100 threads per
+# transform halve the column tiles to 8 transforms, half-line HBM runs: 2D 1000^2 gained nothing.)
 
 # Radices with a register butterfly (butterflies.h): prime, power-of-two and two-factor composites.
 RADICES = [20, 16, 15, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2]
@@ -135,15 +140,18 @@ def radix_sequence(n):
         if ranked:
             ranked.sort(key=lambda cs: cs[0])  # stable: ties keep the larger leading radices
             # tuning builds (SIP_GEN_SEQ_RANK=k) take the k-th sequence by cost among those with as
-            # few passes as the first, for lengths above 1024
+            # few passes as the first
             npass = len(ranked[0][1])
-            if n > 1024 and os.environ.get("SIP_GEN_SEQ_EXTRA") == "1":
-                # tuning builds: the best sequence with one pass more, where one exists
+            # tuning builds vary the lengths in SIP_GEN_SEQ_RANGE ("lo,hi", default 1025..16384)
+            lo, hi = (int(v) for v in os.environ.get("SIP_GEN_SEQ_RANGE", "1025,16384").split(","))
+            tuned = lo <= n <= hi
+            if tuned and os.environ.get("SIP_GEN_SEQ_EXTRA") == "1":
+                # the best sequence with one pass more, where one exists
                 more = [cs for cs in ranked if len(cs[1]) == npass + 1]
                 if more:
                     return more[0][1]
             same = [cs for cs in ranked if len(cs[1]) == npass]
-            rank = int(os.environ.get("SIP_GEN_SEQ_RANK", "0")) if n > 1024 else 0
+            rank = int(os.environ.get("SIP_GEN_SEQ_RANK", "0")) if tuned else 0
             return same[min(rank, len(same) - 1)][1]
     f = factor(n)
     seq = []
