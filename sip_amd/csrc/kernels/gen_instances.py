@@ -87,6 +87,13 @@ SEQ_OVERRIDE = {
     # leaves up to 1024, timed in both four-step roles (n = 256 L, 128 MiB), as rows and as 2D L x L
     720: [12, 12, 5],        # four-step 268 / 237, rows 51.9 / 45.2, 2D 120 / 101
     800: [20, 8, 5],         # four-step 273 / 257, rows 46.1 / 48.8, 2D 124 / 109
+    # long-row leaves (SIP_GEN_BIG_RANK tuning builds; C2C rows, 256 MiB, from 256 rows up)
+    19440: [16, 15, 9, 9],   # 187 / 148
+    20480: [16, 16, 10, 8],  # 158 / 145
+    20736: [16, 16, 9, 9],   # 158 / 148
+    21600: [16, 15, 15, 6],  # 214 / 163
+    25920: [16, 9, 9, 5, 4],  # 206 / 193
+    30000: [10, 10, 10, 6, 5],  # 258 / 215
 }
 # (1000 = 10 x 10 x 10 was faster in the four-step roles, 305 / 283 us, but its 100 threads per
 # transform halve the column tiles to 8 transforms, half-line HBM runs: 2D 1000^2 gained nothing.)
@@ -213,18 +220,17 @@ def big_threads(n, seq):
 def big_radix_sequence(n):
     """Fewest passes over BIG_RADICES within BIG_MAX_ELEMS values per thread, then the fewest idle
     butterfly rounds; largest radices first."""
-    best, best_key = None, None
+    if n in SEQ_OVERRIDE:
+        return list(SEQ_OVERRIDE[n])
+    found = []
 
     def rec(m, maxr, seq):
-        nonlocal best, best_key
         if m == 1:
             t = big_threads(n, seq)
             if elems_per_thread(n, seq, t) > BIG_MAX_ELEMS:
                 return
             waste = sum((-(-(n // r) // t) * t - n // r) / (n // r) for r in seq)
-            key = (len(seq), waste)
-            if best_key is None or key < best_key:
-                best, best_key = list(seq), key
+            found.append(((len(seq), waste), list(seq)))
             return
         if len(seq) >= 8:
             return
@@ -233,7 +239,13 @@ def big_radix_sequence(n):
                 rec(m // r, r, seq + [r])
 
     rec(n, 1 << 30, [])
-    return best  # None: no sequence fits; the length stays a four-step split
+    if not found:
+        return None  # no sequence fits; the length stays a four-step split
+    found.sort(key=lambda ks: ks[0])  # stable: ties keep the larger leading radices
+    # tuning builds (SIP_GEN_BIG_RANK=k): the k-th sequence among those with the fewest passes
+    same = [ks for ks in found if ks[0][0] == found[0][0][0]]
+    rank = int(os.environ.get("SIP_GEN_BIG_RANK", "0"))
+    return same[min(rank, len(same) - 1)][1]
 
 
 def instances():
