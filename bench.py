@@ -303,8 +303,8 @@ def settle(torch, leg):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the batched 1D path shards "
-                                   "by splitting the batch, see DESIGN.md)"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1 (the batched 1D path "
+                                   "would shard by splitting the batch across GPUs, one process each)"}))
         sys.exit(2)
     import torch
     import sip_amd as S
