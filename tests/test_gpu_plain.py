@@ -107,6 +107,28 @@ def test_c2c_single_pass_long_rows(torch_cuda, n):
     assert rel <= TOL
 
 
+def _measured_lengths():
+    import importlib.util
+    import os
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "sip_amd", "csrc", "kernels",
+                        "gen_instances.py")
+    spec = importlib.util.spec_from_file_location("gen_instances_for_test", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return sorted(mod.SEQ_OVERRIDE)
+
+
+@pytest.mark.parametrize("n", _measured_lengths())
+def test_c2c_measured_radix_sequences(torch_cuda, n):
+    """Lengths whose radix sequence was chosen by measurement: forward against the oracle and the
+    round trip, at a batch that runs them as leaves (256 rows from 16384 up)."""
+    b = 256 if n > 16384 else max(2, min(64, (1 << 20) // n))
+    x, y = _c2c_case(n, b, FWD, seed=8800 + n)
+    back = c2c(y, [n], INV) / n
+    rel, _ = report(f"roundtrip {n}", back, x.astype(np.complex128))
+    assert rel <= TOL
+
+
 def test_c2c_2_pow_27(torch_cuda):
     """Largest size the reference accepts (0 < N <= 2^27): three-level four-step."""
     n = 1 << 27

@@ -354,3 +354,20 @@ def test_set_precision_validation():
     finally:
         h.destroy()
     assert S.lib().sipFftSetPrecision(None, 1) == S.ACL_ERROR_INVALID_PARAM
+
+
+def _seq_override():
+    import importlib.util
+    path = os.path.join(ROOT, "sip_amd", "csrc", "kernels", "gen_instances.py")
+    spec = importlib.util.spec_from_file_location("gen_instances_for_test", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.SEQ_OVERRIDE
+
+
+def test_measured_radix_sequences_are_built():
+    """Every length whose radix sequence was chosen by measurement runs that sequence (as a leaf of
+    its own length: from 256 rows up for the lengths above 16384)."""
+    for n, seq in sorted(_seq_override().items()):
+        st = dry([n], S.ASCEND_FFT_C2C, 256)["stages"][0]
+        assert st["L"] == n and st["radices"] == list(seq), (n, st["radices"], seq)
