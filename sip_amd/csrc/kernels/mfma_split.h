@@ -354,8 +354,20 @@ struct Mf2RowIo {
     {
         return tile_base((uint32_t)(tile * a->F + fl), a->tout);
     }
-    __device__ __forceinline__ void store(int fl, int j, float2 v) const { gstore(out + out_base(fl) + j, v); }
-    __device__ __forceinline__ void store_own(int j, float2 v) const { gstore(own_out + j, v); }
+    __device__ __forceinline__ void store(int fl, int j, float2 v) const
+    {
+        if (a->nt_out)
+            gstore_nt(out + out_base(fl) + j, v);
+        else
+            gstore(out + out_base(fl) + j, v);
+    }
+    __device__ __forceinline__ void store_own(int j, float2 v) const
+    {
+        if (a->nt_out)
+            gstore_nt(own_out + j, v);
+        else
+            gstore(own_out + j, v);
+    }
 };
 
 // Single-plane rows kernel with the radix-16 passes on the matrix cores; the first pass reads HBM
@@ -383,7 +395,14 @@ __device__ __forceinline__ void mf2_rows(const StockhamArgs &a, float *pl, int64
     if constexpr (R0 == 16) {
         if constexpr (MfLane<T, false>::FAST) {
             // a wave lies inside one transform: the lane's own row (an inactive wave reads nothing)
-            if (active) {
+            if (active && a.nt_in) {
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+#pragma unroll
+                    for (int tt = 0; tt < 4; tt++)
+                        v[0][4 * u + tt] = gload_nt(io.own_in + ml.b[u] + (4 * ml.g + tt) * NBF0);
+                }
+            } else if (active) {
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
 #pragma unroll
