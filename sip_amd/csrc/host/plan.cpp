@@ -1148,7 +1148,18 @@ void plan_mark_streaming(Plan &p)
         const bool rows_aligned_out = in && in->T % 16 == 0 && (in->L / in->radix[in->npass - 1]) % 16 == 0;
         const bool cols_in = s.sa.col_direct && s.sa.split_lds;
         const bool aligned_in = cols_in ? s.sa.F % 16 == 0 : rows_aligned_in;
-        s.sa.nt_out = (on_out && !read_later && rows_aligned_out) ? 1 : 0;
+        // column outputs (column tiles, rows-in / columns-out): runs of F adjacent transforms at a
+        // stride of tout.es elements; only for outputs of 512 MiB or more of plain (not gathered)
+        // plans: measured on MI355X, 65536 x 2048 823 -> 774 us, 2D 1024^2 x 128 847 -> 808 us,
+        // 512^2 x 256 394 -> 352 us, 3D 128^3 x 64 778 -> 758 us, but the batching form's graph
+        // 8.9 -> 9.5 ms with them on its groups' final stores
+        const bool cols_out = s.sa.tout.unit_f != 0;
+        const bool gathered = !p.row_tables.empty() || !p.external_tables.empty();
+        const int64_t out_bytes = s.sa.nfft * (int64_t)s.sa.out_len * 8;
+        const bool aligned_out = cols_out ? (s.sa.F % 16 == 0 && s.sa.tout.es % 16 == 0 && s.sa.tw4_ntot == 0 &&
+                                             !gathered && out_bytes >= kNtInBytes)
+                                          : rows_aligned_out;
+        s.sa.nt_out = (on_out && !read_later && aligned_out) ? 1 : 0;
         const int64_t in_bytes = s.sa.nfft * (int64_t)s.sa.in_len * 8;
         s.sa.nt_in = (on_in && s.sa.io_mode == IO_C2C && in_bytes >= kNtInBytes && aligned_in) ? 1 : 0;
     }

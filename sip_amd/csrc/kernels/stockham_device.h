@@ -661,8 +661,13 @@ struct ColIoT {
     __device__ __forceinline__ void store_bf(int d, const float2 (&v)[R]) const
     {
         if (TW == 0 || (TW == 2 && a->tw4_ntot == 0)) {
+            if (a->nt_out) {
 #pragma unroll
-            for (int r = 0; r < R; r++) gstore(out + (int64_t)(d + r * NS) * es_out, v[r]);
+                for (int r = 0; r < R; r++) gstore_nt(out + (int64_t)(d + r * NS) * es_out, v[r]);
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) gstore(out + (int64_t)(d + r * NS) * es_out, v[r]);
+            }
             return;
         }
         const int64_t step = col * (int64_t)NS;

@@ -252,14 +252,16 @@ def test_long_rows_without_single_pass_sequence(n):
 
 
 def test_streaming_access_choices():
-    """Nontemporal stores on a stage whose output no later stage reads; nontemporal loads on C2C
-    stages that read 512 MiB or more."""
+    """Nontemporal stores on a stage whose output no later stage reads (column outputs only from
+    512 MiB up); nontemporal loads on C2C stages that read 512 MiB or more."""
     st = dry([4096], S.ASCEND_FFT_C2C, 16384)["stages"][0]          # 512 MiB in
     assert st["nt_out"] == 1 and st["nt_in"] == 1
     st = dry([4096], S.ASCEND_FFT_C2C, 16383)["stages"][0]          # just below
     assert st["nt_out"] == 1 and st["nt_in"] == 0
-    d = dry([1024, 1024], S.ASCEND_FFT_C2C, 8)                      # rows then columns in place
-    assert [s["nt_out"] for s in d["stages"]] == [0, 1] and [s["nt_in"] for s in d["stages"]] == [0, 0]
+    d = dry([1024, 1024], S.ASCEND_FFT_C2C, 8)                      # rows then columns in place, 64 MiB
+    assert [s["nt_out"] for s in d["stages"]] == [0, 0] and [s["nt_in"] for s in d["stages"]] == [0, 0]
+    d = dry([1024, 1024], S.ASCEND_FFT_C2C, 128)                    # 1 GiB
+    assert [s["nt_out"] for s in d["stages"]] == [0, 1] and [s["nt_in"] for s in d["stages"]] == [1, 1]
     d = dry([65536], S.ASCEND_FFT_C2C, 2048)                        # four-step, 1 GiB per stage
     assert [s["nt_out"] for s in d["stages"]] == [0, 1] and [s["nt_in"] for s in d["stages"]] == [1, 1]
     st = dry([8192], S.ASCEND_FFT_R2C, 1 << 16)["stages"][0]        # real rows keep cached loads
