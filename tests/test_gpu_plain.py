@@ -8,7 +8,8 @@ import numpy as np
 import pytest
 
 import oracle
-from fftcase import FWD, INV, c2c, c2r, r2c, report, sign_of
+import sip_amd as S
+from fftcase import FWD, INV, c2c, c2r, make, r2c, report, sign_of, to_dev
 
 pytestmark = pytest.mark.gpu
 
@@ -127,6 +128,34 @@ def test_c2c_measured_radix_sequences(torch_cuda, n):
     back = c2c(y, [n], INV) / n
     rel, _ = report(f"roundtrip {n}", back, x.astype(np.complex128))
     assert rel <= TOL
+
+
+@pytest.mark.parametrize("dims,b", [([65536], 2048), ([1024, 1024], 128), ([512, 512], 256), ([128, 128, 128], 64)])
+def test_streaming_column_outputs_1gib(torch_cuda, monkeypatch, dims, b):
+    """1 GiB C2C plans whose last stage stores column tiles nontemporally: bit-equal to the same
+    transform with cached stores (SIP_FFT_NT_OUT=0 at plan time), and three batch entries against
+    the oracle."""
+    torch = torch_cuda
+    x = oracle.uniform_complex((b, *dims), 0x5EED0A00 + len(dims))
+    xd = to_dev(x)
+    outs = []
+    for nt in ("1", "0"):
+        monkeypatch.setenv("SIP_FFT_NT_OUT", nt)
+        h = make(dims, S.ASCEND_FFT_C2C, FWD, b)
+        try:
+            stages = h.describe()["stages"]
+            assert stages[-1]["nt_out"] == (1 if nt == "1" else 0) and len(stages) >= 2
+            yd = torch.empty_like(xd)
+            assert h.exec_c2c(xd, yd) == S.ACL_SUCCESS and h.synchronize() == S.ACL_SUCCESS
+            outs.append(yd)
+        finally:
+            h.destroy()
+    assert torch.equal(outs[0], outs[1]), "nontemporal stores changed the result"
+    for k in (0, b // 2, b - 1):
+        got = outs[0][k].cpu().numpy()
+        ref = oracle.c2c_nd(x[k], len(dims), oracle.FORWARD)
+        rel, _ = report(f"c2c {dims} x {b} entry {k}", got, ref)
+        assert rel <= TOL
 
 
 def test_c2c_2_pow_27(torch_cuda):
