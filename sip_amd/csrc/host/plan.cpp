@@ -1118,7 +1118,8 @@ void plan_materialize_external(Plan &p, const std::vector<const int64_t *> &dev_
 }
 
 // Streaming (nontemporal) HBM accesses per launch. Stores: the last pass of a stage whose output no
-// later stage of the plan reads (the single-plane C2C rows kernel honours it). Loads: the first pass
+// later stage of the plan reads (single-plane rows, the matrix-core rows, and column tiles under
+// the conditions below). Loads: the first pass
 // of a stage that reads 512 MiB or more (single-plane rows, column tiles, rows-in / columns-out),
 // far beyond the 256 MB last-level cache. Measured on MI355X (one box, in-process A/B against plain
 // accesses): nontemporal stores 4096 x 16384 C2C 192 -> 184 us and the batching workload 9.2 ->

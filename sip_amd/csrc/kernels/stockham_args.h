@@ -74,7 +74,8 @@ struct StockhamArgs {
     float *out_im;         // split-plane output: real parts at (float *)out, imaginary parts here
     int32_t in_sep;        // the input is two float planes (element offsets as for complex)
     int32_t out_sep;       // the output is two float planes
-    int32_t nt_out;        // single-plane C2C rows: the last pass stores nontemporal
+    int32_t nt_out;        // single-plane rows / matrix-core rows / column tiles without the
+                           // four-step twiddle: the last pass stores nontemporal
     int32_t nt_in;         // single-plane rows / column tiles / rows-in columns-out: the first pass
                            // loads nontemporal
     FastDiv div_in_len;    // divides by in_len
