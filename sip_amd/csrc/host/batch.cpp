@@ -30,6 +30,11 @@
 namespace sip {
 namespace {
 
+// Rows from which a group of length 16384 < n <= 32768 runs as one single-pass launch (plain plans:
+// 256). Few rows leave a lone launch short of the chip, but the graph's concurrent chains fill it:
+// measured on MI355X, the batching workload's graph 8.60 -> 8.48 ms against 256 (16 rows: 8.49).
+constexpr int64_t kBatchLeafRows = 64;
+
 struct Group {
     int64_t n = 0;
     std::vector<int> members;
@@ -114,7 +119,7 @@ std::string tune_key(int64_t n, int64_t rows, int direction)
     int64_t bucket = 1;
     while (bucket < rows) bucket <<= 1;
     std::ostringstream os;
-    os << "c2c:" << n << ":" << bucket << ":" << direction << ":" << device_name();
+    os << "c2c.2:" << n << ":" << bucket << ":" << direction << ":" << device_name();
     return os.str();
 }
 
@@ -185,6 +190,7 @@ std::unique_ptr<Plan> build_group_plan(const BatchPlan &bp, const Group &g, cons
     auto p = std::make_unique<Plan>();
     p->direction = bp.direction;
     p->tune = t;
+    p->big_leaf_rows = kBatchLeafRows;
     // element offsets relative to the lowest member pointer of each side
     uintptr_t in_lo = UINTPTR_MAX, out_lo = UINTPTR_MAX;
     for (int i : g.members) {
@@ -218,6 +224,7 @@ std::unique_ptr<Plan> build_tuning_plan(const BatchPlan &bp, const Group &g, con
     auto p = std::make_unique<Plan>();
     p->direction = bp.direction;
     p->tune = t;
+    p->big_leaf_rows = kBatchLeafRows;
     std::vector<int64_t> rows((size_t)g.rows);
     for (int64_t r = 0; r < g.rows; r++) rows[(size_t)r] = r * g.n;
     plan_build_gathered_c2c(*p, g.n, rows, rows, scratch_in, scratch_out);
@@ -312,9 +319,21 @@ void tune_group(BatchPlan &bp, Group &g, const TuneScratch &sc)
             return;
         }
     }
+    std::vector<Tune> cands = candidates(g.n);
+    // A length above 16384 that the default plan runs as one single-pass launch keeps it. Alone,
+    // the four-step split can time faster (its two launches fill the chip where a few hundred
+    // one-workgroup transforms do not), but inside the graph the concurrent chains fill the chip
+    // and the split's second HBM pass costs more: measured on MI355X, the batching workload
+    // 8.92 -> 8.56 ms with the 11 such groups the timing had split kept single-pass.
+    if (g.n > 16384) {
+        try {
+            if (build_tuning_plan(bp, g, Tune{}, sc.in, sc.out)->stages.size() == 1) cands.assign(1, Tune{});
+        } catch (const SipError &) {
+        }
+    }
     Tune best;
     float best_us = 1e30f;
-    for (const Tune &t : candidates(g.n)) {
+    for (const Tune &t : cands) {
         std::unique_ptr<Plan> p;
         try {
             p = build_tuning_plan(bp, g, t, sc.in, sc.out);

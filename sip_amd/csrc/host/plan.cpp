@@ -354,11 +354,12 @@ bool env_flag(const char *name, bool dflt)
 constexpr int64_t kColumnLeafMax = 1024;
 constexpr int64_t kRowLeafMax = 16384;
 // single-pass leaves above kRowLeafMax: transforms per launch from which they replace the split
-// (tuning switch SIP_FFT_BIG_LEAF_ROWS)
-static int64_t big_leaf_min_rows()
+// (the plan's own choice, else 256; the tuning switch SIP_FFT_BIG_LEAF_ROWS overrides both)
+static int64_t big_leaf_min_rows(const Plan &p)
 {
     const char *e = std::getenv("SIP_FFT_BIG_LEAF_ROWS");
-    return (e && e[0]) ? std::atoll(e) : 256;
+    if (e && e[0]) return std::atoll(e);
+    return p.big_leaf_rows > 0 ? p.big_leaf_rows : 256;
 }
 constexpr int64_t kFillWorkgroups = 512;  // two workgroups per CU of MI355X (256 CUs)
 
@@ -403,7 +404,7 @@ struct Builder {
         // 1.39 ms, 30375 x 4418 1.11 / 1.55 ms; with few transforms the split's parallelism wins
         // (32768 x 16 23 / 14 us, 20000 x 64 25 / 20 us).
         return inst->rows_fwd != nullptr && in.es == 1 && out.es == 1 && !is_user(in.buf) && !is_user(out.buf) &&
-               p.precision == PREC_FP32 && p.big_leaf && count(normalize(in.dims)) >= big_leaf_min_rows() &&
+               p.precision == PREC_FP32 && p.big_leaf && count(normalize(in.dims)) >= big_leaf_min_rows(p) &&
                env_flag("SIP_FFT_BIG_LEAF", true);
     }
 

@@ -122,6 +122,9 @@ struct Plan {
     // must not depend on how many rows share a launch (the serving form's coalescing) turns them
     // off: the four-step split is then taken at every row count.
     bool big_leaf = true;
+    // Transforms per launch from which those leaves replace the split; 0 takes the plain-plan
+    // default (256). The batching form's groups run concurrently on several chains and set 64.
+    int64_t big_leaf_rows = 0;
     // ISTFT plan (asdFftIstftMakePlan): inverse C2C of the (channel, n_fft, frames) spectra into a
     // frame workspace, then the windowed overlap-add into (channel, out_len)
     struct Istft {

@@ -2,8 +2,9 @@
 
 Checks: every problem against the FP64 oracle (rel-L2 <= 2e-6); the batched result is bit-identical
 to each problem run alone through asdFftExecC2C when both take the same plan (same kernels, same
-per-transform arithmetic; the problems here keep every group of lengths 16384..32768 below the 256
-rows from which such a group runs as a single-pass leaf instead of a four-step split);
+per-transform arithmetic; the problems here keep every group of lengths 16384..32768 below the 64
+rows from which such a group runs as a single-pass leaf in a batch instead of a four-step split,
+256 in a plain plan);
 hipGraph replays (with eager work and synchronisation in between) reproduce the eager result bit
 for bit; autotuned plans stay within tolerance and reuse the on-disk cache.
 """
