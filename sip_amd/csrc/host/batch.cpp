@@ -586,7 +586,9 @@ int32_t sipFftBatchMakePlan(void *handle, int32_t count, const int64_t *lengths,
         // (one item whose cost is its groups' bytes)
         double pcost = 0;
         for (const sip::Group &g : pgroups) pcost += g.cost;
-        size_t max_chains = 4;  // tuning switch SIP_FFT_BATCH_CHAINS
+        // parallel chains (tuning switch SIP_FFT_BATCH_CHAINS): measured on MI355X, the batching
+        // workload's graph 8.37-8.52 ms on 4 chains, 8.17-8.24 on 8, no better on 12 or 16
+        size_t max_chains = 8;
         if (const char *e = std::getenv("SIP_FFT_BATCH_CHAINS")) max_chains = (size_t)std::max(1, std::min(16, std::atoi(e)));
         b->nchains = (int)std::min<size_t>(max_chains, b->groups.size() + (pgroups.empty() ? 0 : 1));
         std::vector<double> load((size_t)b->nchains, 0.0);
